@@ -1,0 +1,189 @@
+"""Benchmark: incoherent sparse-voxel-DAG traversal on one MI355X.
+
+Workload (pathtrace_1024): the 1024^3 volume (seed 1, 6 fBm octaves, 2048 spheres, 9 materials)
+and 2^24 incoherent rays (seed 7: uniform origins in empty voxels, uniform directions), FP32,
+intersectVolume with surface properties on and LOD off. One step = one traversal launch over all
+2^24 rays; rays, node store and result buffer are resident in HBM before timing starts.
+
+Prints one JSON line (rank 0). `roofline` is computed from the dominant kernel's own HIP-event
+durations on its launch stream; algorithmic bytes per ray are 24 (ray) + 32 (record) + 32 x node
+entries (each entry reads one 32-byte node), with node entries counted by the kernel (checked
+bit-exact against the oracle by the parity tests). `cpu_baseline` times the compiled reference
+(oracle/_ref; the C restatement where that is absent) on all host cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    n = int(env) if env and env.isdigit() else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch from the committed rocprof PMC summary, if one has been recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic_intersect_incoherent_1024.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(hv, rays, hits_gpu, cq):
+    """Reference CPU raycast on the host cores over a bounded sample of the same rays."""
+    from oracle.bindings import Oracle, Reference
+
+    threads = cpu_threads()
+    kind = "reference" if Reference.available() else "port"
+    with tempfile.TemporaryDirectory() as d:
+        if kind == "reference":
+            path = os.path.join(d, "v.dag")
+            hv.save_dag(path)
+            rv = Reference().load(path)
+            timer = lambda r, reps: rv.time_intersect(r, True, -1.0, threads=threads, reps=reps)
+            check = lambda r: rv.intersect(r, True)
+        else:
+            orc = Oracle()
+            sub = orc.find_subdags(hv.nodes, hv.root)
+            timer = lambda r, reps: orc.time_intersect(hv.nodes, sub, r, True, -1.0, threads=threads, reps=reps)
+            check = lambda r: orc.to_results(r, orc.intersect(hv.nodes, sub, r, True, threads=threads))
+        t_cal, _ = timer(rays[: 1 << 16], 1)
+        rate = (1 << 16) / max(t_cal, 1e-9)
+        sample = int(min(rays.shape[0], max(1 << 18, rate * 3.0)))  # ~3 s per repetition
+        t, nhit = timer(rays[:sample], 3)
+        ns = 1 << 15
+        ref = check(rays[:ns])
+        mine = cq.hits_to_intersections(rays[:ns], hits_gpu[:ns], True)
+        same = bool((ref.view(np.uint8).reshape(ns, -1) == mine.view(np.uint8).reshape(ns, -1)).all())
+    return {
+        "value": round(sample / t / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"first {sample} of the 2^24 incoherent rays, {threads} threads, 4096-ray interleaved chunks, "
+                  f"best of 3 after 1 warm-up",
+        "gpu_records_bitexact_on_first": ns if same else 0,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard by "
+              f"replicating the node store per GPU; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import cubiquity_amd as cq
+    from cubiquity_amd import workloads as W
+
+    w = W.PATHTRACE_1024
+    t0 = time.time()
+    hv = cq.HostVolume.generate(w.params())
+    rays = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, args.rays)
+    log(f"volume {w.side}^3: {hv.node_count} nodes ({hv.node_count * 32 / 1e6:.1f} MB), "
+        f"{args.rays} rays, built in {time.time() - t0:.2f}s")
+
+    ctx = cq.Context(0)
+    dvol = ctx.upload(hv)
+    d_rays = torch.from_numpy(rays).cuda()
+    d_hits = torch.empty((args.rays, 32), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    cfg = cq.LaunchCfg(variant=args.variant)
+
+    for _ in range(args.warmup):
+        cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=cfg, out=d_hits)
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=cfg, out=d_hits)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    kernel_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = float(np.mean(kernel_ms))
+
+    hits = d_hits.cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
+    entries = int(hits["entries"].astype(np.int64).sum())
+    iters = int(hits["iters"].astype(np.int64).sum())
+    anomalies = int(((hits["info"] >> 15) & 1).sum())
+    alg_bytes = args.rays * (24 + 32) + 32 * entries
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+
+    value = args.rays * args.steps / wall / 1e6
+    out = {
+        "metric": "Mrays/s incoherent SVDAG traversal @1024^3; adjoint/forward time ratio",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "pathtrace_1024 volume (1024^3, seed 1) + 2^24 incoherent rays (seed 7), "
+                        "intersectVolume surface properties on, LOD off",
+            "rays": args.rays,
+            "nodes": hv.node_count,
+            "variant": args.variant,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel_ms": round(avg_ms, 4),
+            "bytes_per_launch": alg_bytes,
+            "node_entries_per_ray": round(entries / args.rays, 3),
+            "iterations_per_ray": round(iters / args.rays, 3),
+        },
+        "hit_fraction": round(float((hits["info"] & 1).mean()), 5),
+        "anomalies": anomalies,
+        "adjoint_forward_ratio": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(hv, rays, hits, cq)
+        out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,264 @@
+"""cubiquity_amd: MI355X-native sparse-voxel-DAG ray traversal (Cubiquity's raycast path).
+
+Python host layer over the C ABI of ``libcqx.so`` (``include/cqx.h``). It mirrors the
+reference's raycast API (``findSubDAGs``, ``intersectVolume``, ``Volume``,
+src/library/raytracing.h:57-81, storage.h:135-197) with batched, device-resident calls.
+Torch is used only for device memory and streams. Loading fails loudly if the library has not
+been built (``python -m cubiquity_amd.build``); there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import cabi
+from .cabi import (CQX_SURFACE_PROPERTIES, MAX_FOOTPRINT_DISABLED, CqxError, GenParams, LaunchCfg,
+                   HIT_DTYPE, INTERSECTION_DTYPE, SUBDAG_DTYPE, lib, check)
+
+__all__ = [
+    "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
+    "incoherent_rays", "gen_default_params", "hits_to_intersections", "CqxError", "GenParams",
+    "LaunchCfg", "HIT_DTYPE", "INTERSECTION_DTYPE", "SUBDAG_DTYPE", "CQX_SURFACE_PROPERTIES",
+    "MAX_FOOTPRINT_DISABLED", "decode_info",
+]
+
+
+def gen_default_params(log2_size, seed, **overrides):
+    """Scaled generator defaults for a 2^log2_size volume (cqx_gen_default_params)."""
+    p = GenParams()
+    check(lib().cqx_gen_default_params(int(log2_size), ctypes.c_uint64(seed), ctypes.byref(p)))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class HostVolume:
+    """Host node store + root: the library's view of a reference ``Volume``."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def generate(cls, params, threads=0):
+        h = ctypes.c_void_p()
+        check(lib().cqx_hv_generate(ctypes.byref(params), int(threads), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_dag(cls, path):
+        h = ctypes.c_void_p()
+        check(lib().cqx_hv_load_dag(os.fsencode(path), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_dense(cls, grid, lower=(0, 0, 0)):
+        """Canonical DAG of a dense uint8 grid indexed [z][y][x] whose voxel (0,0,0) sits at `lower`."""
+        grid = np.ascontiguousarray(grid, dtype=np.uint8)
+        if grid.ndim != 3:
+            raise ValueError("grid must be 3-D, indexed [z][y][x]")
+        size = (ctypes.c_int32 * 3)(grid.shape[2], grid.shape[1], grid.shape[0])
+        lo = (ctypes.c_int32 * 3)(*[int(v) for v in lower])
+        h = ctypes.c_void_p()
+        check(lib().cqx_hv_from_dense(grid.ctypes.data_as(ctypes.c_void_p), size, lo, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_nodes(cls, nodes, root):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+        h = ctypes.c_void_p()
+        check(lib().cqx_hv_from_nodes(nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      ctypes.c_uint64(nodes.shape[0]), ctypes.c_uint32(root), ctypes.byref(h)))
+        return cls(h)
+
+    def save_dag(self, path):
+        check(lib().cqx_hv_save_dag(self._h, os.fsencode(path)))
+
+    @property
+    def node_count(self):
+        return int(lib().cqx_hv_node_count(self._h))
+
+    @property
+    def root(self):
+        return int(lib().cqx_hv_root(self._h))
+
+    @property
+    def nodes(self):
+        """Read-only numpy view (node_count, 8) of the store, valid while this object lives."""
+        n = self.node_count
+        ptr = lib().cqx_hv_nodes(self._h)
+        arr = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_uint32)), shape=(n * 8,))
+        arr = arr.reshape(n, 8)
+        arr.flags.writeable = False
+        return arr
+
+    def voxel(self, x, y, z):
+        m = ctypes.c_uint32()
+        check(lib().cqx_hv_voxel(self._h, int(x), int(y), int(z), ctypes.byref(m)))
+        return int(m.value)
+
+    def find_subdags(self):
+        return findSubDAGs(self.nodes, self.root)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and cabi.loaded():
+            lib().cqx_hv_free(h)
+
+
+def findSubDAGs(nodes, root):
+    """findSubDAGs (src/library/raytracing.cpp:89-97) over a raw (n, 8) uint32 store."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+    out = np.zeros(8, dtype=SUBDAG_DTYPE)
+    check(lib().cqx_find_subdags(nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                 ctypes.c_uint64(nodes.shape[0]), ctypes.c_uint32(root),
+                                 out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def camera_rays(eye, target, fov_y_degrees, width, height):
+    """Pinhole rays (camera.cpp:12-38 convention), float32 array (height*width, 6)."""
+    out = np.empty((width * height, 6), dtype=np.float32)
+    e = (ctypes.c_double * 3)(*eye)
+    t = (ctypes.c_double * 3)(*target)
+    check(lib().cqx_camera_rays(e, t, ctypes.c_double(fov_y_degrees), ctypes.c_uint32(width),
+                                ctypes.c_uint32(height), out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def incoherent_rays(volume, log2_size, seed, n, threads=0):
+    """Uniform origins in empty voxels of [0,N)^3, uniform directions; float32 (n, 6)."""
+    out = np.empty((n, 6), dtype=np.float32)
+    check(lib().cqx_incoherent_rays(volume._h, int(log2_size), ctypes.c_uint64(seed), ctypes.c_uint64(n),
+                                    int(threads), out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def hits_to_intersections(rays, hits, surface_properties=True):
+    """Expand compact records into RayVolumeIntersection fields (raytracing.h:48-55)."""
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+    hits = np.ascontiguousarray(hits).view(HIT_DTYPE).reshape(-1)
+    out = np.zeros(hits.shape[0], dtype=INTERSECTION_DTYPE)
+    check(lib().cqx_hits_to_intersections(rays.ctypes.data_as(ctypes.c_void_p), hits.ctypes.data_as(ctypes.c_void_p),
+                                          ctypes.c_uint64(hits.shape[0]),
+                                          ctypes.c_uint32(CQX_SURFACE_PROPERTIES if surface_properties else 0),
+                                          out.ctypes.data_as(ctypes.c_void_p)))
+    return out
+
+
+def decode_info(info):
+    """Split the packed ``info`` word of hit records into named numpy fields."""
+    info = np.asarray(info, dtype=np.uint32)
+    return {
+        "hit": (info & 1).astype(bool),
+        "slot": (info >> 1) & 7,
+        "size_log2": (info >> 4) & 31,
+        "normal_code": (info >> 9) & 63,
+        "anomaly": ((info >> 15) & 1).astype(bool),
+        "material": (info >> 16) & 255,
+    }
+
+
+class DeviceVolume:
+    """A node store resident in HBM (cqx_volume_upload)."""
+
+    def __init__(self, ctx, handle, node_count, root):
+        self._ctx = ctx
+        self._h = handle
+        self.node_count = node_count
+        self.root = root
+
+    def info(self):
+        nodes = ctypes.c_void_p()
+        sub = np.zeros(8, dtype=SUBDAG_DTYPE)
+        check(lib().cqx_volume_info(self._h, ctypes.byref(nodes), None, None, sub.ctypes.data_as(ctypes.c_void_p)))
+        return nodes.value, sub
+
+    def free(self):
+        if self._h and self._ctx._h:
+            check(lib().cqx_volume_free(self._ctx._h, self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A device context (cqx_create) on one GPU."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().cqx_create(int(device), ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def upload(self, volume, subdags=None):
+        if isinstance(volume, HostVolume):
+            nodes, root = volume.nodes, volume.root
+        else:
+            nodes, root = volume
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+        sub_ptr = None
+        if subdags is not None:
+            subdags = np.ascontiguousarray(subdags, dtype=SUBDAG_DTYPE)
+            sub_ptr = subdags.ctypes.data_as(ctypes.c_void_p)
+        h = ctypes.c_void_p()
+        check(lib().cqx_volume_upload(self._h, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      ctypes.c_uint64(nodes.shape[0]), ctypes.c_uint32(root), sub_ptr,
+                                      ctypes.byref(h)))
+        return DeviceVolume(self, h, nodes.shape[0], root)
+
+    def intersect(self, dvol, rays_ptr, n, hits_ptr, flags=CQX_SURFACE_PROPERTIES,
+                  max_footprint=MAX_FOOTPRINT_DISABLED, cfg=None, stream=None):
+        """Raw batched call on device pointers (ints)."""
+        cfgp = ctypes.byref(cfg) if cfg is not None else None
+        check(lib().cqx_intersect(self._h, dvol._h, ctypes.c_void_p(rays_ptr), ctypes.c_uint64(n),
+                                  ctypes.c_uint32(flags), ctypes.c_float(max_footprint), ctypes.c_void_p(hits_ptr),
+                                  cfgp, ctypes.c_void_p(stream or 0)))
+
+    def intersect_host(self, dvol, rays, flags=CQX_SURFACE_PROPERTIES, max_footprint=MAX_FOOTPRINT_DISABLED):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        check(lib().cqx_intersect_host(self._h, dvol._h, rays.ctypes.data_as(ctypes.c_void_p),
+                                       ctypes.c_uint64(rays.shape[0]), ctypes.c_uint32(flags),
+                                       ctypes.c_float(max_footprint), hits.ctypes.data_as(ctypes.c_void_p)))
+        return hits
+
+    def close(self):
+        if self._h:
+            check(lib().cqx_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def intersectVolume(ctx, dvol, rays, computeSurfaceProperties=True, maxFootprint=MAX_FOOTPRINT_DISABLED,
+                    cfg=None, stream=None, out=None):
+    """Batched intersectVolume (src/library/raytracing.cpp:397-478) on device-resident rays.
+
+    ``rays`` is a torch tensor (n, 6) float32 on the GPU; returns (or fills ``out``) a torch
+    uint8 tensor of n 32-byte records (view with ``HIT_DTYPE`` after ``.cpu().numpy()``).
+    """
+    import torch
+
+    if not (isinstance(rays, torch.Tensor) and rays.is_cuda and rays.dtype == torch.float32):
+        raise TypeError("rays must be a float32 GPU tensor of shape (n, 6)")
+    rays = rays.contiguous().view(-1, 6)
+    n = rays.shape[0]
+    if out is None:
+        out = torch.empty((n, 32), dtype=torch.uint8, device=rays.device)
+    if out.numel() < n * 32:
+        raise ValueError("out too small")
+    if stream is None:
+        stream = torch.cuda.current_stream(rays.device).cuda_stream
+    flags = CQX_SURFACE_PROPERTIES if computeSurfaceProperties else 0
+    ctx.intersect(dvol, rays.data_ptr(), n, out.data_ptr(), flags, maxFootprint, cfg, stream)
+    return out

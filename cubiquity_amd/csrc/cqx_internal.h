@@ -1,0 +1,43 @@
+// Internal declarations shared by the host-side sources of libcqx.so.
+#pragma once
+
+#include "cqx.h"
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cqx {
+
+// Records `code` and a printf-style message as the calling thread's last error; returns code.
+cqx_status fail(cqx_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+constexpr uint32_t kMaterialCount = CQX_MATERIAL_COUNT;
+constexpr int kRootHeight = 32;
+
+struct Node {
+    uint32_t c[8];
+};
+static_assert(sizeof(Node) == 32, "node is 32 bytes");
+
+// Host node store + root: the library's view of a reference Volume.
+struct HostVolume {
+    std::vector<Node> nodes; // includes the 256 material nodes
+    uint32_t root = 0;
+};
+
+// A NodeStore holding only the 256 material nodes (storage.cpp:110-122).
+std::vector<Node> material_nodes();
+
+cqx_status validate_dag(const uint32_t* nodes, uint64_t node_count, uint32_t root);
+cqx_status find_subdags(const uint32_t* nodes, uint64_t node_count, uint32_t root, cqx_subdag out[8]);
+
+int default_threads();
+
+} // namespace cqx
+
+struct cqx_host_volume {
+    cqx::HostVolume v;
+};

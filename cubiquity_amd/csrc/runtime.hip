@@ -1,0 +1,321 @@
+// libcqx device runtime: contexts, device node stores, and the traversal kernels for gfx950.
+//
+// Kernels
+//   k_intersect_lane        one ray per lane, one pass over the ray array (variant 1)
+//   k_intersect_persistent  persistent waves that keep every lane busy: a lane whose ray is done
+//                           stores its record and, once enough lanes of the wave are idle,
+//                           the wave claims a contiguous block of new rays from a global counter
+//                           with one atomic and hands them out by ballot prefix (variant 2)
+// Both keep the traversal stack in LDS (one column per lane, one row per node height).
+#include "cqx_internal.h"
+#include "traverse_core.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+using namespace cqxd;
+
+struct cqx_ctx {
+    int device = 0;
+    int cu_count = 0;
+    int lds_per_block = 0;
+    uint64_t* d_counter = nullptr; // work counter of the persistent kernel
+    void* scratch = nullptr;       // staging for cqx_intersect_host
+    size_t scratch_bytes = 0;
+};
+
+struct cqx_dvolume {
+    uint32_t* nodes = nullptr;
+    cqx_subdag* sub = nullptr;
+    uint64_t node_count = 0;
+    uint32_t root = 0;
+    cqx_subdag hsub[8];
+    int max_height = 0; // largest SubDAG start height (stack rows needed: max_height + 1)
+};
+
+namespace {
+
+using cqx::fail;
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) return fail(CQX_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t kDefaultMaxIters = 1u << 22;
+
+// Best-effort release paths: an error here has nowhere useful to go.
+inline void release(void* p) { (void)hipFree(p); }
+
+// ---- variant 1: one ray per lane --------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_intersect_lane(TraceParams P, const cqx_ray* __restrict__ rays, uint64_t n,
+                                                        cqx_hit* __restrict__ hits)
+{
+    extern __shared__ uint32_t lds_stack[];
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = reinterpret_cast<const float*>(rays + i);
+    const float ox = r[0], oy = r[1], oz = r[2], dx = r[3], dy = r[4], dz = r[5];
+    cqx_hit h;
+    trace_ray(P, ox, oy, oz, dx, dy, dz, lds_stack + threadIdx.x, blockDim.x, h);
+    hits[i] = h;
+}
+
+// ---- variant 2: persistent waves with ballot refill ------------------------------------------
+// Every wave owns 64 lanes for the whole launch. Between bursts of `regroup` traversal steps the
+// wave ballots its idle lanes; once at least `refill` lanes are idle (or all are), one lane claims
+// that many consecutive ray ids from the wave's queue with a single atomic and the idle lanes take
+// them in lane order (ballot prefix). The ray array is split into one queue per XCD
+// (blockIdx.x % 8, the order in which workgroups are spread over the XCDs), so waves of one XCD
+// draw from their own counter; a wave whose queue is drained moves on to the next queue. A wave
+// exits when every queue is drained and all its lanes are idle.
+constexpr int kQueues = 8;
+
+__global__ void k_reset_counters(unsigned long long* counters)
+{
+    if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, const cqx_ray* __restrict__ rays,
+                                                              uint64_t n, cqx_hit* __restrict__ hits,
+                                                              unsigned long long* __restrict__ counters,
+                                                              uint32_t regroup, uint32_t refill)
+{
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t* stack = lds_stack + threadIdx.x;
+    const uint32_t stride = blockDim.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t per_q = (n + kQueues - 1) / kQueues;
+    uint32_t qc = blockIdx.x % kQueues; // current queue
+    uint32_t drained = 0;               // queues found empty so far
+
+    RayState S;
+    S.phase = kIdle;
+    uint64_t rid = 0;
+    cqx_hit out;
+    bool exhausted = false;
+
+    for (;;) {
+        const uint64_t idle = __ballot(S.phase == kIdle);
+        const uint32_t nidle = uint32_t(__popcll(idle));
+        if (exhausted && nidle == 64) break;
+        if (!exhausted && (nidle >= refill || nidle == 64)) {
+            const uint64_t q_begin = per_q * qc;
+            const uint64_t q_end = q_begin + per_q < n ? q_begin + per_q : n;
+            const uint64_t q_len = q_end > q_begin ? q_end - q_begin : 0;
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(&counters[qc], (unsigned long long)nidle);
+            base = __shfl(base, 0);
+            if (base + nidle >= q_len) {
+                qc = (qc + 1) % kQueues;
+                if (++drained == kQueues) exhausted = true;
+            }
+            if (S.phase == kIdle) {
+                const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                const uint64_t k = base + rank;
+                if (k < q_len) {
+                    rid = q_begin + k;
+                    const float* r = reinterpret_cast<const float*>(rays + rid);
+                    ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
+                }
+            }
+        }
+        for (uint32_t s = 0; s < regroup; s++) {
+            if (S.phase != kIdle) {
+                const StepResult res = (S.phase == kOctant) ? octant_enter(P, S, stack, stride, out)
+                                                            : walk_step(P, S, stack, stride, out);
+                if (res == kDone) hits[rid] = out;
+            }
+        }
+    }
+}
+
+cqx_status check_ctx(cqx_ctx* ctx)
+{
+    if (!ctx) return fail(CQX_ERR_INVALID_ARG, "null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return CQX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+cqx_status cqx_create(int device, cqx_ctx** out)
+{
+    if (!out) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CQX_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(CQX_ERR_NO_DEVICE, "device %d not present (%d devices)", device, count);
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    cqx_ctx* c = new (std::nothrow) cqx_ctx;
+    if (!c) return fail(CQX_ERR_OOM, "out of host memory");
+    c->device = device;
+    c->cu_count = prop.multiProcessorCount;
+    c->lds_per_block = int(prop.sharedMemPerBlock);
+    if (hipMalloc(&c->d_counter, sizeof(uint64_t) * 64) != hipSuccess) {
+        delete c;
+        return fail(CQX_ERR_OOM, "device allocation of the work counters failed");
+    }
+    *out = c;
+    return CQX_OK;
+}
+
+cqx_status cqx_destroy(cqx_ctx* ctx)
+{
+    if (!ctx) return CQX_OK;
+    (void)hipSetDevice(ctx->device);
+    release(ctx->d_counter);
+    release(ctx->scratch);
+    delete ctx;
+    return CQX_OK;
+}
+
+cqx_status cqx_volume_upload(cqx_ctx* ctx, const uint32_t* nodes, uint64_t node_count, uint32_t root,
+                             const cqx_subdag sub[8], cqx_dvolume** out)
+{
+    if (!out) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    st = cqx::validate_dag(nodes, node_count, root);
+    if (st != CQX_OK) return st;
+    cqx_subdag s[8];
+    if (sub) {
+        std::memcpy(s, sub, sizeof(s));
+        for (int c = 0; c < 8; c++)
+            if (s[c].node >= node_count || s[c].height < 0 || s[c].height > 31)
+                return fail(CQX_ERR_INVALID_DAG, "SubDAG %d invalid (node %u, height %d)", c, s[c].node, s[c].height);
+    } else {
+        st = cqx::find_subdags(nodes, node_count, root, s);
+        if (st != CQX_OK) return st;
+    }
+    cqx_dvolume* v = new (std::nothrow) cqx_dvolume;
+    if (!v) return fail(CQX_ERR_OOM, "out of host memory");
+    v->node_count = node_count;
+    v->root = root;
+    std::memcpy(v->hsub, s, sizeof(s));
+    v->max_height = 0;
+    for (int c = 0; c < 8; c++)
+        if (s[c].node > 0) v->max_height = std::max(v->max_height, int(s[c].height));
+    const size_t bytes = size_t(node_count) * 32;
+    if (hipMalloc(&v->nodes, bytes) != hipSuccess || hipMalloc(&v->sub, sizeof(s)) != hipSuccess) {
+        release(v->nodes);
+        delete v;
+        return fail(CQX_ERR_OOM, "device allocation of %zu bytes failed", bytes);
+    }
+    if (hipMemcpy(v->nodes, nodes, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(v->sub, s, sizeof(s), hipMemcpyHostToDevice) != hipSuccess) {
+        release(v->nodes);
+        release(v->sub);
+        delete v;
+        return fail(CQX_ERR_HIP, "upload of the node store failed");
+    }
+    *out = v;
+    return CQX_OK;
+}
+
+cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol)
+{
+    if (!vol) return CQX_OK;
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    release(vol->nodes);
+    release(vol->sub);
+    delete vol;
+    return CQX_OK;
+}
+
+cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, uint64_t* node_count, uint32_t* root,
+                           cqx_subdag sub[8])
+{
+    if (!vol) return fail(CQX_ERR_INVALID_ARG, "null volume");
+    if (dev_nodes) *dev_nodes = vol->nodes;
+    if (node_count) *node_count = vol->node_count;
+    if (root) *root = vol->root;
+    if (sub) std::memcpy(sub, vol->hsub, sizeof(vol->hsub));
+    return CQX_OK;
+}
+
+cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
+                         float max_footprint, cqx_hit* hits, const cqx_launch_cfg* cfg, void* stream)
+{
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    if (!vol) return fail(CQX_ERR_INVALID_ARG, "null volume");
+    if (flags & ~CQX_SURFACE_PROPERTIES) return fail(CQX_ERR_INVALID_ARG, "unknown flags 0x%x", flags);
+    if (n == 0) return CQX_OK;
+    if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
+    if (n > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "%llu rays exceed the 2^38 limit", (unsigned long long)n);
+
+    TraceParams P;
+    P.nodes = vol->nodes;
+    P.sub = vol->sub;
+    P.surf = (flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
+    P.max_footprint = max_footprint;
+    P.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : kDefaultMaxIters;
+
+    const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
+    if (block % 64 != 0 || block > 256) return fail(CQX_ERR_INVALID_ARG, "block %u must be a multiple of 64, <= 256", block);
+    const uint32_t rows = uint32_t(vol->max_height) + 1;
+    const size_t lds = size_t(rows) * block * sizeof(uint32_t);
+    if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
+
+    const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
+    if (variant == 1) {
+        const uint64_t blocks = (n + block - 1) / block;
+        hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, hipStream_t(stream), P, rays, n,
+                           hits);
+    } else if (variant == 2) {
+        const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
+        const uint32_t regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
+        const uint32_t refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
+        if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
+            return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
+        const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
+        const uint64_t need_blocks = (n + block - 1) / block;
+        const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, hipStream_t(stream),
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
+        hipLaunchKernelGGL(k_intersect_persistent, dim3(blocks), dim3(block), lds, hipStream_t(stream), P, rays, n, hits,
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
+    } else {
+        return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
+    }
+    HIP_TRY(hipGetLastError());
+    return CQX_OK;
+}
+
+cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
+                              float max_footprint, cqx_hit* hits)
+{
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    if (n == 0) return CQX_OK;
+    if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
+    const size_t need = size_t(n) * (sizeof(cqx_ray) + sizeof(cqx_hit));
+    if (need > ctx->scratch_bytes) {
+        release(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+        if (hipMalloc(&ctx->scratch, need) != hipSuccess) return fail(CQX_ERR_OOM, "device staging of %zu bytes failed", need);
+        ctx->scratch_bytes = need;
+    }
+    cqx_ray* d_rays = static_cast<cqx_ray*>(ctx->scratch);
+    cqx_hit* d_hits = reinterpret_cast<cqx_hit*>(static_cast<char*>(ctx->scratch) + size_t(n) * sizeof(cqx_ray));
+    HIP_TRY(hipMemcpy(d_rays, rays, size_t(n) * sizeof(cqx_ray), hipMemcpyHostToDevice));
+    st = cqx_intersect(ctx, vol, d_rays, n, flags, max_footprint, d_hits, nullptr, nullptr);
+    if (st != CQX_OK) return st;
+    HIP_TRY(hipMemcpy(hits, d_hits, size_t(n) * sizeof(cqx_hit), hipMemcpyDeviceToHost));
+    return CQX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,333 @@
+// Device-side ray vs sparse-voxel-DAG traversal for gfx950: a per-lane restatement of
+// Cubiquity's intersectVolume / intersectRayNodeESVO (src/library/raytracing.cpp:134-478) with the
+// exact FP32 semantics of the reference's Release build:
+//   * IEEE binary32 add/sub/mul and correctly rounded division (built with -ffp-contract=off and
+//     -fhip-fp32-correctly-rounded-divide-sqrt, no denormal flushing);
+//   * std::min / std::max are compare-selects in the reference's operand order (NaN order kept);
+//   * int -> float conversions round to nearest; int32 arithmetic wraps.
+//
+// The traversal is a resumable per-lane state machine so that a persistent wave can interleave
+// traversal steps with refilling finished lanes:
+//   ray_setup      intersectVolume's ray reflection and starting octant (raytracing.cpp:415-434)
+//   octant_enter   SubDAG of the current octant: slab test and ESVO initialisation (:443-458, :220-250)
+//   walk_step      one iteration of the ESVO do/while loop (:253-367)
+//   octant_advance next octant in order of axis crossings (:469-475)
+// The traversal stack lives in LDS, one column per lane: slot h of a lane is stack[h * stride].
+#pragma once
+
+#include "cqx.h"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cqxd {
+
+constexpr uint32_t kMaterialCount = 256u;
+constexpr float kFltMax = 3.402823466e+38f;
+
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float min3f(float x, float y, float z) { return smin(smin(x, y), z); }
+__device__ __forceinline__ float max3f(float x, float y, float z) { return smax(smax(x, y), z); }
+
+__device__ __forceinline__ int find_msb(uint32_t v) { return v ? 31 - __builtin_clz(v) : -1; }
+
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
+__device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return int32_t(uint32_t(a) - uint32_t(b)); }
+__device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
+__device__ __forceinline__ int32_t floor_to(int32_t x, int h)
+{
+    h &= 31;
+    return int32_t(uint32_t(x >> h) << h);
+}
+
+// findFirstChild, raytracing.cpp:178-196; child id bits x=1, y=2, z=4.
+__device__ __forceinline__ uint32_t first_child(float entry, float ox, float oy, float oz, float ix, float iy,
+                                                float iz, int32_t cx, int32_t cy, int32_t cz)
+{
+    const float fx = float(cx), fy = float(cy), fz = float(cz);
+    uint32_t id = ((fx - ox) * ix < entry ? 1u : 0u) | ((fy - oy) * iy < entry ? 2u : 0u) |
+                  ((fz - oz) * iz < entry ? 4u : 0u);
+    if (entry <= 0.0f) id |= (ox >= fx ? 1u : 0u) | (oy >= fy ? 2u : 0u) | (oz >= fz ? 4u : 0u);
+    return id;
+}
+
+// First voxel of the hit block entered by the ray (reflected shifted coordinates).
+__device__ __forceinline__ int32_t entered_voxel(float o, float d, float t, int32_t lo, int32_t size, bool entry_axis)
+{
+    if (entry_axis) return lo;
+    const float q = o + d * t;
+    const float f = floorf(q);
+    const int32_t hi = wadd(lo, size - 1);
+    if (!(f >= float(lo))) return lo;
+    if (f > float(hi)) return hi;
+    int32_t v = int32_t(f);
+    v = v < lo ? lo : v;
+    return v > hi ? hi : v;
+}
+
+struct TraceParams {
+    const uint32_t* __restrict__ nodes; // node_count x 8
+    const cqx_subdag* __restrict__ sub; // 8 SubDAGs
+    uint32_t surf;                      // computeSurfaceProperties
+    float max_footprint;
+    uint32_t max_iters;
+};
+
+enum : uint32_t { kIdle = 0, kOctant = 1, kWalk = 2 };
+enum StepResult : uint32_t { kContinue = 0, kDone = 1 };
+
+struct RayState {
+    float ox, oy, oz;       // reflected, shifted origin
+    float dx, dy, dz;       // |direction|
+    float ix, iy, iz;       // 1 / |direction| (the same in every ESVO call)
+    float distx, disty, distz;
+    uint32_t sign_bits;
+    uint32_t oct;
+    uint32_t node;
+    int32_t height, start, csize;
+    uint32_t cid;
+    int32_t cpx, cpy, cpz;
+    float last_exit;
+    uint32_t entries, iters;
+    uint32_t phase;
+};
+
+__device__ __forceinline__ void result_miss(cqx_hit& out)
+{
+    out.t = 0.0f;
+    out.voxel[0] = out.voxel[1] = out.voxel[2] = 0;
+    out.node = 0;
+    out.info = 0;
+}
+
+__device__ __forceinline__ void finish(RayState& S, cqx_hit& out, bool anomaly)
+{
+    if (anomaly) {
+        result_miss(out);
+        out.info = CQX_HIT_ANOMALY;
+    }
+    out.entries = S.entries;
+    out.iters = S.iters;
+    S.phase = kIdle;
+}
+
+// Ray setup, raytracing.cpp:415-434.
+__device__ __forceinline__ void ray_setup(RayState& S, float rox, float roy, float roz, float rdx, float rdy, float rdz)
+{
+    const uint32_t sbx = rdx < 0.0f, sby = rdy < 0.0f, sbz = rdz < 0.0f;
+    S.sign_bits = sbx | (sby << 1) | (sbz << 2);
+    const float sx = float(int32_t(sbx) * -2 + 1), sy = float(int32_t(sby) * -2 + 1), sz = float(int32_t(sbz) * -2 + 1);
+    S.ox = (rox + 0.5f) * sx;
+    S.oy = (roy + 0.5f) * sy;
+    S.oz = (roz + 0.5f) * sz;
+    S.dx = fabsf(rdx);
+    S.dy = fabsf(rdy);
+    S.dz = fabsf(rdz);
+    S.ix = 1.0f / S.dx;
+    S.iy = 1.0f / S.dy;
+    S.iz = 1.0f / S.dz;
+    S.distx = (-S.ox) / S.dx;
+    S.disty = (-S.oy) / S.dy;
+    S.distz = (-S.oz) / S.dz;
+    uint32_t oct = 0;
+    if (S.distx < 0.0f) { oct += 1; S.distx += kFltMax; }
+    if (S.disty < 0.0f) { oct += 2; S.disty += kFltMax; }
+    if (S.distz < 0.0f) { oct += 4; S.distz += kFltMax; }
+    S.oct = oct;
+    S.entries = 0;
+    S.iters = 0;
+    S.phase = kOctant;
+}
+
+// Next octant in order of axis crossings, raytracing.cpp:469-475. Returns kDone when the walk is
+// over (all octants visited: a miss; or no axis steps: the reference would loop forever).
+__device__ __forceinline__ StepResult octant_advance(RayState& S, cqx_hit& out)
+{
+    const float m = min3f(S.distx, S.disty, S.distz);
+    uint32_t step = 0;
+    if (S.distx <= m) { step += 1; S.distx += kFltMax; }
+    if (S.disty <= m) { step += 2; S.disty += kFltMax; }
+    if (S.distz <= m) { step += 4; S.distz += kFltMax; }
+    if (step == 0) {
+        finish(S, out, true);
+        return kDone;
+    }
+    S.oct += step;
+    if (S.oct > 7) {
+        result_miss(out);
+        finish(S, out, false);
+        return kDone;
+    }
+    S.phase = kOctant;
+    return kContinue;
+}
+
+// Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
+// (:220-250), or move on to the next octant.
+__device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                                   cqx_hit& out)
+{
+    const cqx_subdag sd = P.sub[S.oct ^ S.sign_bits];
+    if (sd.node > 0) {
+        const int32_t sbx = S.sign_bits & 1u, sby = (S.sign_bits >> 1) & 1u, sbz = (S.sign_bits >> 2) & 1u;
+        const int32_t size = int32_t(1u << sd.height);
+        const int32_t px = wsub(wmul(sd.lower[0], 1 - 2 * sbx), wmul(sbx, size));
+        const int32_t py = wsub(wmul(sd.lower[1], 1 - 2 * sby), wmul(sby, size));
+        const int32_t pz = wsub(wmul(sd.lower[2], 1 - 2 * sbz), wmul(sbz, size));
+        const uint32_t node_size = 1u << sd.height;
+        const float nsz = float(node_size);
+        const float entry = max3f((float(px) - S.ox) * S.ix, (float(py) - S.oy) * S.iy, (float(pz) - S.oz) * S.iz);
+        const float exitt = min3f(((float(px) + nsz) - S.ox) * S.ix, ((float(py) + nsz) - S.oy) * S.iy,
+                                  ((float(pz) + nsz) - S.oz) * S.iz);
+        if (entry < exitt) {
+            S.start = sd.height;
+            S.height = sd.height;
+            for (int h = 0; h <= S.start; h++) stack[h * stride] = 0;
+            S.node = sd.node;
+            S.csize = int32_t(node_size / 2);
+            const uint32_t cid = first_child(entry, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(px, S.csize),
+                                             wadd(py, S.csize), wadd(pz, S.csize));
+            S.cid = cid;
+            S.cpx = wadd(px, (cid & 1u) ? S.csize : 0);
+            S.cpy = wadd(py, (cid & 2u) ? S.csize : 0);
+            S.cpz = wadd(pz, (cid & 4u) ? S.csize : 0);
+            S.last_exit = exitt;
+            S.entries++;
+            S.phase = kWalk;
+            return kContinue;
+        }
+    }
+    return octant_advance(S, out);
+}
+
+// One iteration of the ESVO loop, raytracing.cpp:253-367.
+__device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                                cqx_hit& out)
+{
+    if (S.iters >= P.max_iters) {
+        finish(S, out, true);
+        return kDone;
+    }
+    S.iters++;
+    const int32_t csize = S.csize;
+    const float t1x = (float(wadd(S.cpx, csize)) - S.ox) * S.ix;
+    const float t1y = (float(wadd(S.cpy, csize)) - S.oy) * S.iy;
+    const float t1z = (float(wadd(S.cpz, csize)) - S.oz) * S.iz;
+    const float tcx = min3f(t1x, t1y, t1z);
+    const uint32_t slot = S.cid ^ S.sign_bits;
+    const uint32_t child = P.nodes[uint64_t(S.node) * 8 + slot];
+    if (child > 0) {
+        const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
+                    t0z = (float(S.cpz) - S.oz) * S.iz;
+        const float tce = max3f(t0x, t0y, t0z);
+        const bool internal = child >= kMaterialCount;
+        const bool large = (float(csize) / tcx) > P.max_footprint;
+        if (internal && large) {
+            if (S.height <= 1) {
+                finish(S, out, true); // deeper than a valid DAG
+                return kDone;
+            }
+            if (tcx < S.last_exit) stack[S.height * stride] = S.node;
+            S.last_exit = tcx;
+            S.height--;
+            S.node = child;
+            const int32_t half = csize / 2;
+            S.csize = half;
+            S.entries++;
+            const uint32_t cid = first_child(tce, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(S.cpx, half),
+                                             wadd(S.cpy, half), wadd(S.cpz, half));
+            S.cid = cid;
+            S.cpx = wadd(S.cpx, (cid & 1u) ? half : 0);
+            S.cpy = wadd(S.cpy, (cid & 2u) ? half : 0);
+            S.cpz = wadd(S.cpz, (cid & 4u) ? half : 0);
+            return kContinue;
+        }
+        // Hit (raytracing.cpp:302-320). A SubDAG of height 0 (the chain below the root ends in a
+        // single voxel) has child size 0: the hit block is then the SubDAG node itself, of side 1.
+        const int32_t bsize = csize > 0 ? csize : 1;
+        uint32_t info = 1u | (slot << 1) | (uint32_t(find_msb(uint32_t(bsize))) << 4);
+        if (P.surf) {
+            // findNearestMaterial, raytracing.cpp:134-162
+            const uint32_t order = (0x76534210u | 0x88888888u) ^ (S.sign_bits * 0x11111111u);
+            uint32_t m = child;
+            while (m >= kMaterialCount) {
+                S.entries++;
+                uint32_t next = 0;
+                for (uint32_t ids = order; ids != 0; ids >>= 4) {
+                    const uint32_t c = P.nodes[uint64_t(m) * 8 + (ids & 7u)];
+                    if (c > 0) { next = c; break; }
+                }
+                // An internal node without occupied children (never produced by bake) would make
+                // the reference loop forever.
+                if (next == 0) {
+                    finish(S, out, true);
+                    return kDone;
+                }
+                m = next;
+            }
+            // normal = vec3(equal(tEntry, childT0)) * -rayDirSign, raytracing.cpp:317-318
+            const float sx = (S.sign_bits & 1u) ? -1.0f : 1.0f, sy = (S.sign_bits & 2u) ? -1.0f : 1.0f,
+                        sz = (S.sign_bits & 4u) ? -1.0f : 1.0f;
+            const float nx = float(tce == t0x) * -sx, ny = float(tce == t0y) * -sy, nz = float(tce == t0z) * -sz;
+            info |= ((nx != 0.0f ? 1u : 0u) | ((__float_as_uint(nx) >> 31) << 1)) << 9;
+            info |= ((ny != 0.0f ? 1u : 0u) | ((__float_as_uint(ny) >> 31) << 1)) << 11;
+            info |= ((nz != 0.0f ? 1u : 0u) | ((__float_as_uint(nz) >> 31) << 1)) << 13;
+            info |= (m & 0xffu) << 16;
+        }
+        const int32_t vx = entered_voxel(S.ox, S.dx, tce, S.cpx, bsize, t0x == tce);
+        const int32_t vy = entered_voxel(S.oy, S.dy, tce, S.cpy, bsize, t0y == tce);
+        const int32_t vz = entered_voxel(S.oz, S.dz, tce, S.cpz, bsize, t0z == tce);
+        out.voxel[0] = (S.sign_bits & 1u) ? ~vx : vx;
+        out.voxel[1] = (S.sign_bits & 2u) ? ~vy : vy;
+        out.voxel[2] = (S.sign_bits & 4u) ? ~vz : vz;
+        out.t = tce;
+        out.node = S.node;
+        out.info = info;
+        finish(S, out, false);
+        return kDone;
+    }
+    // Advance to the next sibling, raytracing.cpp:328-332.
+    const uint32_t flips = (t1x <= tcx ? 1u : 0u) | (t1y <= tcx ? 2u : 0u) | (t1z <= tcx ? 4u : 0u);
+    const uint32_t cid = S.cid ^ flips;
+    S.cid = cid;
+    const int32_t px = S.cpx, py = S.cpy, pz = S.cpz;
+    S.cpx = wadd(px, (flips & 1u) ? csize : 0);
+    S.cpy = wadd(py, (flips & 2u) ? csize : 0);
+    S.cpz = wadd(pz, (flips & 4u) ? csize : 0);
+    if ((cid & flips) != flips) {
+        // Pop to the highest boundary crossed, raytracing.cpp:339-364.
+        const uint32_t diff = uint32_t(px ^ S.cpx) | uint32_t(py ^ S.cpy) | uint32_t(pz ^ S.cpz);
+        const int msb = find_msb(diff);
+        const int height = msb + 1;
+        S.height = height;
+        if (height > S.start) return octant_advance(S, out); // left the SubDAG without a hit
+        S.node = stack[height * stride];
+        const int32_t ns = int32_t(1u << msb);
+        S.csize = ns;
+        const uint32_t bx = uint32_t(S.cpx >> msb) & 1u, by = uint32_t(S.cpy >> msb) & 1u,
+                       bz = uint32_t(S.cpz >> msb) & 1u;
+        S.cid = bx | (by << 1) | (bz << 2);
+        S.cpx = wadd(floor_to(S.cpx, height), bx ? ns : 0);
+        S.cpy = wadd(floor_to(S.cpy, height), by ? ns : 0);
+        S.cpz = wadd(floor_to(S.cpz, height), bz ? ns : 0);
+        S.last_exit = 0.0f;
+        S.entries++;
+    }
+    return kContinue;
+}
+
+// Runs one ray to completion (state machine driven in a loop).
+__device__ __forceinline__ void trace_ray(const TraceParams& P, float rox, float roy, float roz, float rdx, float rdy,
+                                          float rdz, uint32_t* stack, uint32_t stride, cqx_hit& out)
+{
+    RayState S;
+    ray_setup(S, rox, roy, roz, rdx, rdy, rdz);
+    for (;;) {
+        const StepResult r = (S.phase == kOctant) ? octant_enter(P, S, stack, stride, out)
+                                                  : walk_step(P, S, stack, stride, out);
+        if (r == kDone) break;
+    }
+}
+
+} // namespace cqxd

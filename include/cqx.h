@@ -1,0 +1,219 @@
+/* cqx.h: C ABI of the MI355X (gfx950) sparse-voxel-DAG traversal library (libcqx.so).
+ *
+ * Drop-in boundary for Cubiquity's raycast path. Plain pointers and sizes, no C++ types,
+ * every function returns a cqx_status (0 = OK, < 0 = error) and never throws; the message of
+ * the last error on the calling thread is available from cqx_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the Cubiquity checkout):
+ *   SubDAG / findSubDAGs              src/library/raytracing.h:57-69, raytracing.cpp:43-97
+ *   intersectVolume (per ray)         src/library/raytracing.h:71-76, raytracing.cpp:397-478
+ *   RayVolumeIntersection             src/library/raytracing.h:48-55
+ *   NodeStore raw bytes + SubDAGArray upload to the GPU
+ *                                     src/application/commands/view/gpu_pathtracing_viewer.cpp:43-57
+ *   Volume::load / save (.dag)        src/library/storage.cpp:505-542, 192-206
+ *   Volume::voxel                     src/library/storage.cpp:481-499
+ *
+ * Threading: one cqx_ctx per host thread, or calls on a context serialized by the caller.
+ * Host buffers are caller-owned; device volumes are owned by the context that uploaded them.
+ */
+#ifndef CQX_H
+#define CQX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t cqx_status;
+#define CQX_OK 0
+#define CQX_ERR_INVALID_ARG (-1) /* null pointer, bad size or flag */
+#define CQX_ERR_INVALID_DAG (-2) /* child index out of range, cycle, deeper than 32 levels */
+#define CQX_ERR_HIP (-3)         /* a HIP runtime call failed */
+#define CQX_ERR_OOM (-4)         /* host or device allocation failed */
+#define CQX_ERR_IO (-5)          /* file could not be read or written */
+#define CQX_ERR_NO_DEVICE (-6)   /* no usable GPU */
+#define CQX_ERR_LIMIT (-7)       /* input exceeds a documented limit */
+
+/* Number of material ids; node indices below this are material (leaf) nodes whose children
+ * all point at themselves. Matches Internals::MaterialCount, src/library/storage.h:55. */
+#define CQX_MATERIAL_COUNT 256u
+
+/* == SubDAG, src/library/raytracing.h:57-65 (32 B, same field order and padding). */
+typedef struct {
+    int32_t lower[3]; /* lower corner in signed voxel coordinates */
+    int32_t height;   /* node height (side = 2^height voxels) */
+    uint32_t pad0;
+    uint32_t node;    /* node index; 0 = empty octant */
+    uint32_t pad1, pad2;
+} cqx_subdag;
+
+/* One ray, FP32, 24 B: the six floats intersectVolume takes (raytracing.h:72-75). */
+typedef struct {
+    float o[3];
+    float d[3];
+} cqx_ray;
+
+/* One traversal result, 32 B (the GPU's compact record).
+ *   t        RayVolumeIntersection::distance (an FP32 value); 0 on miss
+ *   voxel    first voxel of the hit block entered by the ray, signed volume coordinates
+ *   node     index of the node whose child was hit
+ *   info     bit 0 hit | bits 1-3 child slot in `node` | bits 4-8 log2 of the hit block's side
+ *            | bits 9-14 normal code (axis i: bit 9+2i set if |n_i| == 1, bit 10+2i = signbit)
+ *            | bit 15 anomaly (iteration cap reached or octant walk stalled; reported as miss)
+ *            | bits 16-23 material (0 unless CQX_SURFACE_PROPERTIES)
+ *   entries  node entries (SubDAG roots entered, descents, pops inside the SubDAG, nearest-
+ *            material steps); each reads one 32-B node
+ *   iters    traversal loop iterations (child reads)
+ * The reference's position is o + d * t (FP32 multiply then add) and its normal is decoded from
+ * the normal code; cqx_hits_to_intersections() produces both. */
+typedef struct {
+    float t;
+    int32_t voxel[3];
+    uint32_t node;
+    uint32_t info;
+    uint32_t entries;
+    uint32_t iters;
+} cqx_hit;
+
+#define CQX_HIT_FLAG 0x1u
+#define CQX_HIT_ANOMALY 0x8000u
+
+/* == RayVolumeIntersection, src/library/raytracing.h:48-55, as plain fields. */
+typedef struct {
+    int32_t hit;
+    uint32_t material;
+    double distance;
+    float position[3];
+    float normal[3];
+} cqx_intersection;
+
+/* intersect flags */
+#define CQX_SURFACE_PROPERTIES 0x1u /* computeSurfaceProperties (material + normal) */
+
+/* reference MAX_FOOTPRINT_DISABLED, src/library/raytracing.h:71 */
+#define CQX_MAX_FOOTPRINT_DISABLED (-1.0f)
+
+/* ---- errors ---------------------------------------------------------------------------------- */
+const char* cqx_last_error(void);
+const char* cqx_version(void);
+
+/* ---- host node store (replaces Volume/NodeStore access on the path) --------------------------- */
+
+/* Nodes are stored as in the reference's NodeStore: node_count x uint32[8], indices 0..255 are
+ * the material nodes {i,i,i,i,i,i,i,i}, internal nodes start at 256 (storage.cpp:110-122). */
+typedef struct cqx_host_volume cqx_host_volume;
+
+/* Volume::load, storage.cpp:505-528 + NodeStore::read, storage.cpp:192-199 */
+cqx_status cqx_hv_load_dag(const char* path, cqx_host_volume** out);
+/* Volume::save's file layout, storage.cpp:530-542 (the store is written as it is, not re-baked) */
+cqx_status cqx_hv_save_dag(const cqx_host_volume* hv, const char* path);
+/* Copy a raw store (incl. the 256 material nodes) and validate it. */
+cqx_status cqx_hv_from_nodes(const uint32_t* nodes, uint64_t node_count, uint32_t root,
+                             cqx_host_volume** out);
+void cqx_hv_free(cqx_host_volume* hv);
+const uint32_t* cqx_hv_nodes(const cqx_host_volume* hv);
+uint64_t cqx_hv_node_count(const cqx_host_volume* hv);
+uint32_t cqx_hv_root(const cqx_host_volume* hv);
+/* Volume::voxel, storage.cpp:481-499 */
+cqx_status cqx_hv_voxel(const cqx_host_volume* hv, int32_t x, int32_t y, int32_t z, uint32_t* material);
+
+/* findSubDAGs, raytracing.cpp:89-97, over a raw store. Validates every index it follows. */
+cqx_status cqx_find_subdags(const uint32_t* nodes, uint64_t node_count, uint32_t root,
+                            cqx_subdag out[8]);
+
+/* Checks that every child index is < node_count, that the DAG reachable from `root` is acyclic
+ * and at most 32 levels deep (root at height 32) and that node_count fits the 32-bit indices. */
+cqx_status cqx_validate_dag(const uint32_t* nodes, uint64_t node_count, uint32_t root);
+
+/* Expand compact records into reference RayVolumeIntersection fields (host). */
+cqx_status cqx_hits_to_intersections(const cqx_ray* rays, const cqx_hit* hits, uint64_t n,
+                                     uint32_t flags, cqx_intersection* out);
+
+/* ---- seeded synthetic workloads (the benchmark's volumes and rays) --------------------------- */
+
+/* Heightfield + spheres generator. Voxel (x,y,z) of [0, 2^log2_size)^3 is
+ *   sphere material  if inside any sphere (last sphere wins): (x-cx)^2+(y-cy)^2+(z-cz)^2 <= r^2
+ *   2 (rock)         else if y < height(x,z) and y < rock_height
+ *   1 (ground)       else if y < height(x,z)
+ *   0 (empty)        otherwise
+ * height(x,z) = base_height + amplitude * fbm(x/period, z/period), fbm = normalised sum of
+ * `octaves` octaves of 2D value noise (gain, lacunarity). Sphere k: centre U[0,N)^3,
+ * radius U[r_min, r_max], material 3 + k mod (n_materials - 2), drawn from PCG32(seed, stream
+ * 0x5fe7e5). Everything is evaluated in FP64. */
+typedef struct {
+    int32_t log2_size;
+    int32_t octaves;
+    int32_t n_spheres;
+    int32_t n_materials; /* material ids used: 1..n_materials (>= 3) */
+    uint64_t seed;
+    double gain, lacunarity, period;
+    double base_height, amplitude, rock_height;
+    double r_min, r_max;
+} cqx_gen_params;
+
+/* The scaled defaults for a side of 2^log2_size (s = N/128): period 64 s, base 64 s,
+ * amplitude 40 s, rock below 40 s, octaves 4 + round(2/3 log2 s), 32 s^2 spheres with radius
+ * [4, 16] s^(1/3), gain 0.5, lacunarity 2, 3 materials. */
+cqx_status cqx_gen_default_params(int32_t log2_size, uint64_t seed, cqx_gen_params* out);
+/* Build the volume as a deduplicated DAG (the shape Volume::bake produces, storage.cpp:208-290),
+ * placed at [0, N)^3. `threads` <= 0 uses all cores. */
+cqx_status cqx_hv_generate(const cqx_gen_params* p, int threads, cqx_host_volume** out);
+
+/* Build the canonical (baked) DAG of a dense grid of material ids, x fastest then y then z,
+ * whose voxel (0,0,0) sits at signed voxel coordinates `lower` (any placement, including across
+ * the origin). The counterpart of filling a Volume with setVoxel and calling bake
+ * (storage.cpp:396-438, 387-394). */
+cqx_status cqx_hv_from_dense(const uint8_t* grid, const int32_t size[3], const int32_t lower[3],
+                             cqx_host_volume** out);
+
+/* Pinhole camera rays through pixel centres, y down, built in FP64 and rounded to FP32
+ * (the convention of src/application/commands/view/camera.cpp:12-38). Row-major, width*height. */
+cqx_status cqx_camera_rays(const double eye[3], const double target[3], double fov_y_degrees,
+                           uint32_t width, uint32_t height, cqx_ray* out);
+/* Incoherent rays: ray i draws from PCG32(seed, i): origin U[0,N)^3 - 0.5 (the volume's extent,
+ * voxel v spanning [v-0.5, v+0.5)), re-drawn until the voxel at floor(origin + 0.5) is empty;
+ * direction uniform on the sphere (z = 2u-1, phi = 2 pi v), computed in FP64, rounded to FP32. */
+cqx_status cqx_incoherent_rays(const cqx_host_volume* hv, int32_t log2_size, uint64_t seed,
+                               uint64_t n, int threads, cqx_ray* out);
+
+/* ---- device ------------------------------------------------------------------------------------ */
+
+typedef struct cqx_ctx cqx_ctx;
+typedef struct cqx_dvolume cqx_dvolume;
+
+cqx_status cqx_create(int device, cqx_ctx** out);
+cqx_status cqx_destroy(cqx_ctx* ctx);
+
+/* Upload a store (node_count x 32 B incl. the material nodes, as the reference's SSBO upload
+ * does) and its SubDAGs (NULL: computed here). Validates the store first. */
+cqx_status cqx_volume_upload(cqx_ctx* ctx, const uint32_t* nodes, uint64_t node_count, uint32_t root,
+                             const cqx_subdag sub[8], cqx_dvolume** out);
+cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol);
+/* Device pointer to the volume's node array (node_count x 8 uint32) and its node count. */
+cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, uint64_t* node_count,
+                           uint32_t* root, cqx_subdag sub[8]);
+
+/* Launch parameters of the traversal kernel. Zero fields take the tuned or built-in defaults. */
+typedef struct {
+    uint32_t variant;          /* 0 auto, 1 one ray per lane, 2 persistent with wave refill */
+    uint32_t block;            /* threads per workgroup (multiple of 64) */
+    uint32_t waves_per_cu;     /* persistent grid size: CUs x this many waves */
+    uint32_t regroup_interval; /* persistent: traversal steps between refill checks */
+    uint32_t refill_lanes;     /* persistent: refill once at least this many lanes are idle */
+    uint32_t max_iters;        /* per-ray iteration cap (0 = built-in 1<<22) */
+} cqx_launch_cfg;
+
+/* intersectVolume over n rays, batched: rays and hits are DEVICE pointers; stream is a
+ * hipStream_t (NULL = default stream). Asynchronous with respect to the host. */
+cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                         uint32_t flags, float max_footprint, cqx_hit* hits,
+                         const cqx_launch_cfg* cfg, void* stream);
+/* Same with host buffers: copies in, runs, copies out, synchronizes. */
+cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                              uint32_t flags, float max_footprint, cqx_hit* hits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CQX_H */

@@ -1,0 +1,249 @@
+"""Test infrastructure only: ctypes bindings of the oracle (liboracle.so, the C restatement) and of
+the compiled reference (_ref/libcubiquity_ref.so). Only tests/, the smoke check and bench.py's
+CPU-baseline leg import this module; the product never does.
+
+Both libraries are built by ``make -C oracle`` (the reference one only where the reference
+checkout exists; the built file travels with the tree).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libcubiquity_ref.so")
+REF_SRC = "/root/reference/src/library"
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("voxel", "<i4", (3,)), ("node", "<u4"), ("info", "<u4"),
+                      ("entries", "<u4"), ("iters", "<u4")])
+RESULT_DTYPE = np.dtype([("hit", "<i4"), ("material", "<u4"), ("distance", "<f8"),
+                         ("position", "<f4", (3,)), ("normal", "<f4", (3,))])
+SUBDAG_DTYPE = np.dtype([("lower", "<i4", (3,)), ("height", "<i4"), ("pad0", "<u4"), ("node", "<u4"),
+                         ("pad1", "<u4"), ("pad2", "<u4")])
+
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+
+def build(ref=True):
+    targets = ["oracle"] + (["ref"] if ref and os.path.isdir(REF_SRC) else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    """The C restatement (cpu_ref.c / cpu_gen.c)."""
+
+    def __init__(self):
+        if not os.path.exists(ORACLE_LIB):
+            build(ref=False)
+        L = ctypes.CDLL(ORACLE_LIB)
+        sig = {
+            "orc_find_subdags": (ctypes.c_int, [_P, _U64, _U32, _P]),
+            "orc_intersect": (None, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_float, _U32, _P]),
+            "orc_intersect_mt": (None, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_float, _U32, _P, ctypes.c_int]),
+            "orc_hits_to_results": (None, [_P, _P, _U64, ctypes.c_int, _P]),
+            "orc_trace_ref": (None, [_P, _U32, _P, _U64, _P]),
+            "orc_voxel": (_U32, [_P, _U32, _I32, _I32, _I32]),
+            "orc_descend": (None, [_P, _U32, _I32, _I32, _I32, _P]),
+            "orc_time_intersect": (ctypes.c_double, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                                     _U64, ctypes.c_int, _P]),
+            "orc_dense_brute": (None, [_P, ctypes.c_int, _P, _U64, _P, _P, _P, _P, _P]),
+            "orc_gen_height": (ctypes.c_double, [_P, _I64, _I64]),
+            "orc_gen_spheres": (None, [_P, _P]),
+            "orc_gen_material": (_U32, [_P, _P, _I64, _I64, _I64]),
+            "orc_gen_dense": (None, [_P, _P, ctypes.c_int]),
+            "orc_compare_dense": (_U64, [_P, ctypes.c_int, _P, _U32, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+
+    @staticmethod
+    def _nodes(nodes):
+        return np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 8)
+
+    def find_subdags(self, nodes, root):
+        nodes = self._nodes(nodes)
+        out = np.zeros(8, dtype=SUBDAG_DTYPE)
+        if self.L.orc_find_subdags(_ptr(nodes), nodes.shape[0], root, _ptr(out)) != 0:
+            raise ValueError("malformed store")
+        return out
+
+    def intersect(self, nodes, subdags, rays, surf=True, max_footprint=-1.0, max_iters=0, threads=0):
+        nodes = self._nodes(nodes)
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        subdags = np.ascontiguousarray(subdags, dtype=SUBDAG_DTYPE)
+        out = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        if threads and threads > 1:
+            self.L.orc_intersect_mt(_ptr(nodes), _ptr(subdags), _ptr(rays), rays.shape[0], int(surf),
+                                    ctypes.c_float(max_footprint), max_iters, _ptr(out), int(threads))
+        else:
+            self.L.orc_intersect(_ptr(nodes), _ptr(subdags), _ptr(rays), rays.shape[0], int(surf),
+                                 ctypes.c_float(max_footprint), max_iters, _ptr(out))
+        return out
+
+    def to_results(self, rays, hits, surf=True):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        hits = np.ascontiguousarray(hits).view(HIT_DTYPE).reshape(-1)
+        out = np.zeros(hits.shape[0], dtype=RESULT_DTYPE)
+        self.L.orc_hits_to_results(_ptr(rays), _ptr(hits), hits.shape[0], int(surf), _ptr(out))
+        return out
+
+    def trace_ref(self, nodes, root, rays):
+        nodes = self._nodes(nodes)
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros(rays.shape[0], dtype=RESULT_DTYPE)
+        self.L.orc_trace_ref(_ptr(nodes), root, _ptr(rays), rays.shape[0], _ptr(out))
+        return out
+
+    def voxel(self, nodes, root, x, y, z):
+        nodes = self._nodes(nodes)
+        return int(self.L.orc_voxel(_ptr(nodes), root, int(x), int(y), int(z)))
+
+    def descend(self, nodes, root, x, y, z):
+        nodes = self._nodes(nodes)
+        path = np.zeros(33, dtype=np.uint32)
+        self.L.orc_descend(_ptr(nodes), root, int(x), int(y), int(z), _ptr(path))
+        return path
+
+    def time_intersect(self, nodes, subdags, rays, surf=True, max_footprint=-1.0, threads=1, chunk=4096, reps=3):
+        nodes = self._nodes(nodes)
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        subdags = np.ascontiguousarray(subdags, dtype=SUBDAG_DTYPE)
+        hits = ctypes.c_uint64()
+        t = self.L.orc_time_intersect(_ptr(nodes), _ptr(subdags), _ptr(rays), rays.shape[0], int(surf),
+                                      ctypes.c_float(max_footprint), int(threads), chunk, reps, ctypes.byref(hits))
+        return t, hits.value
+
+    def dense_brute(self, grid, rays):
+        grid = np.ascontiguousarray(grid, dtype=np.uint8)
+        n = grid.shape[0]
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        m = rays.shape[0]
+        hit = np.zeros(m, np.int32)
+        tn = np.zeros(m, np.float64)
+        ts = np.zeros(m, np.float64)
+        mat = np.zeros(m, np.uint32)
+        vox = np.zeros((m, 3), np.int32)
+        self.L.orc_dense_brute(_ptr(grid), n, _ptr(rays), m, _ptr(hit), _ptr(tn), _ptr(ts), _ptr(mat), _ptr(vox))
+        return hit.astype(bool), tn, ts, mat, vox
+
+    def gen_dense(self, params, threads=8):
+        n = 1 << params.log2_size
+        grid = np.zeros((n, n, n), dtype=np.uint8)  # [z][y][x]
+        self.L.orc_gen_dense(ctypes.byref(params), _ptr(grid), int(threads))
+        return grid
+
+    def gen_height(self, params, x, z):
+        return self.L.orc_gen_height(ctypes.byref(params), int(x), int(z))
+
+    def compare_dense(self, grid, log2_size, nodes, root, threads=8):
+        nodes = self._nodes(nodes)
+        grid = np.ascontiguousarray(grid, dtype=np.uint8)
+        return int(self.L.orc_compare_dense(_ptr(grid), int(log2_size), _ptr(nodes), root, int(threads)))
+
+
+class Reference:
+    """The unmodified reference library compiled in place (oracle/_ref)."""
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_LIB)
+
+    def __init__(self):
+        L = ctypes.CDLL(REF_LIB)
+        sig = {
+            "ref_volume_load": (_P, [ctypes.c_char_p]),
+            "ref_volume_new": (_P, []),
+            "ref_volume_free": (None, [_P]),
+            "ref_volume_root": (_U32, [_P]),
+            "ref_volume_set_voxel": (None, [_P, _I32, _I32, _I32, _U32]),
+            "ref_volume_save": (None, [_P, ctypes.c_char_p]),
+            "ref_volume_bake": (None, [_P]),
+            "ref_volume_count_nodes": (_U32, [_P]),
+            "ref_volume_voxel": (_U32, [_P, _I32, _I32, _I32]),
+            "ref_find_subdags": (None, [_P, _P]),
+            "ref_intersect": (None, [_P, _P, _U64, ctypes.c_int, ctypes.c_float, _P]),
+            "ref_trace_ref": (None, [_P, _P, _U64, _P]),
+            "ref_time_intersect": (ctypes.c_double, [_P, _P, _U64, ctypes.c_int, ctypes.c_float, ctypes.c_int, _U64,
+                                                     ctypes.c_int, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        self.L = L
+
+    def load(self, path):
+        h = self.L.ref_volume_load(os.fsencode(path))
+        if not h:
+            raise IOError(path)
+        return RefVolume(self, h)
+
+    def new(self):
+        return RefVolume(self, self.L.ref_volume_new())
+
+
+class RefVolume:
+    def __init__(self, ref, h):
+        self.ref = ref
+        self.h = h
+
+    def __del__(self):
+        if self.h:
+            self.ref.L.ref_volume_free(self.h)
+            self.h = None
+
+    @property
+    def root(self):
+        return int(self.ref.L.ref_volume_root(self.h))
+
+    def set_voxel(self, x, y, z, m):
+        self.ref.L.ref_volume_set_voxel(self.h, int(x), int(y), int(z), int(m))
+
+    def save(self, path):
+        self.ref.L.ref_volume_save(self.h, os.fsencode(path))
+
+    def bake(self):
+        self.ref.L.ref_volume_bake(self.h)
+
+    def count_nodes(self):
+        return int(self.ref.L.ref_volume_count_nodes(self.h))
+
+    def voxel(self, x, y, z):
+        return int(self.ref.L.ref_volume_voxel(self.h, int(x), int(y), int(z)))
+
+    def find_subdags(self):
+        out = np.zeros(8, dtype=SUBDAG_DTYPE)
+        self.ref.L.ref_find_subdags(self.h, _ptr(out))
+        return out
+
+    def intersect(self, rays, surf=True, max_footprint=-1.0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros(rays.shape[0], dtype=RESULT_DTYPE)
+        self.ref.L.ref_intersect(self.h, _ptr(rays), rays.shape[0], int(surf), ctypes.c_float(max_footprint), _ptr(out))
+        return out
+
+    def trace_ref(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros(rays.shape[0], dtype=RESULT_DTYPE)
+        self.ref.L.ref_trace_ref(self.h, _ptr(rays), rays.shape[0], _ptr(out))
+        return out
+
+    def time_intersect(self, rays, surf=True, max_footprint=-1.0, threads=1, chunk=4096, reps=3):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        hits = ctypes.c_uint64()
+        t = self.ref.L.ref_time_intersect(self.h, _ptr(rays), rays.shape[0], int(surf), ctypes.c_float(max_footprint),
+                                          int(threads), chunk, reps, ctypes.byref(hits))
+        return t, hits.value

@@ -1,0 +1,181 @@
+"""GPU parity: the HIP traversal (through the C ABI) against the oracle, the golden fixtures made
+by the compiled reference, and the compiled reference itself where it is present.
+
+Bar: every field of every 32-byte record bit-identical (t bits, voxel, node, slot, block size,
+normal code incl. signed zeros, material, node-entry and iteration counts)."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import records_equal
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "raycast_128_golden.npz")
+
+
+def _gpu(ctx, dvol, rays, surf=True, fp=-1.0, cfg=None):
+    import torch
+
+    from cubiquity_amd import HIT_DTYPE, intersectVolume
+
+    r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+    out = intersectVolume(ctx, dvol, r, computeSurfaceProperties=surf, maxFootprint=fp, cfg=cfg)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().view(HIT_DTYPE).reshape(-1)
+
+
+@pytest.fixture(scope="module")
+def dvol128(gpu_ctx, vol128):
+    return gpu_ctx.upload(vol128)
+
+
+def _report(name, eq, rays, g, o):
+    if not eq.all():
+        i = np.nonzero(~eq)[0][:4]
+        return f"{name}: {(~eq).sum()} of {len(eq)} differ; first {i.tolist()}\nrays {rays[i]}\ngpu {g[i]}\noracle {o[i]}"
+    return ""
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("surf,fp", [(True, -1.0), (False, -1.0), (True, 0.05), (True, 0.0035)])
+@pytest.mark.parametrize("set_name", ["camera", "incoherent", "special"])
+def test_gpu_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, rays128, set_name, surf, fp, variant):
+    rays = rays128[set_name]
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    o = oracle.intersect(vol128.nodes, sub, rays, surf, fp, threads=8)
+    g = _gpu(gpu_ctx, dvol128, rays, surf, fp, cq.LaunchCfg(variant=variant))
+    eq = records_equal(g, o)
+    assert eq.all(), _report(set_name, eq, rays, g, o)
+
+
+def test_gpu_matches_golden(cq, gpu_ctx, dvol128, vol128):
+    """Against the compiled reference's own outputs stored as fixtures (no reference needed)."""
+    z = np.load(GOLDEN, allow_pickle=False)
+    assert hashlib.sha256(vol128.nodes.tobytes()).hexdigest() == str(z["dag_sha256"])
+    rays = z["rays"]
+    for surf in (1, 0):
+        g = _gpu(gpu_ctx, dvol128, rays, bool(surf))
+        res = cq.hits_to_intersections(rays, g, bool(surf))
+        ref = z[f"ref_surf{surf}"]
+        eq = records_equal(res, ref)
+        assert eq.all(), f"surf={surf}: {(~eq).sum()} rays differ from the reference fixture"
+    # full 65,536-ray camera set: SHA-256 of the reference records
+    cam = z["camera_rays"]
+    g = _gpu(gpu_ctx, dvol128, cam, True)
+    res = cq.hits_to_intersections(cam, g, True)
+    assert hashlib.sha256(res.tobytes()).hexdigest() == str(z["camera_ref_sha256"])
+
+
+def test_gpu_matches_reference_live(cq, reference, gpu_ctx, dvol128, dag128, rays128):
+    rv = reference.load(dag128)
+    for name in ("camera", "incoherent", "special"):
+        rays = rays128[name]
+        g = _gpu(gpu_ctx, dvol128, rays, True)
+        res = cq.hits_to_intersections(rays, g, True)
+        ref = rv.intersect(rays, True)
+        eq = records_equal(res, ref)
+        assert eq.all(), f"{name}: {(~eq).sum()} differ from the compiled reference"
+
+
+def _small_volumes(cq):
+    from tests.test_oracle_pinning import _blob_grid
+
+    g1 = np.zeros((16, 16, 16), np.uint8)
+    g1[4, 4, 4] = 9
+    g3 = np.zeros((4, 4, 4), np.uint8)
+    return {
+        "single_voxel": (cq.HostVolume.from_dense(g1, (0, 0, 0)), 16, (0, 0, 0)),
+        "solid_cube": (cq.HostVolume.from_dense(np.full((8, 8, 8), 3, np.uint8), (0, 0, 0)), 8, (0, 0, 0)),
+        "empty": (cq.HostVolume.from_dense(g3, (0, 0, 0)), 4, (0, 0, 0)),
+        "span_origin": (cq.HostVolume.from_dense(_blob_grid(40, 3), (-17, -23, -11)), 40, (-17, -23, -11)),
+        "negative_octant": (cq.HostVolume.from_dense(_blob_grid(32, 8), (-32, -32, -32)), 32, (-32, -32, -32)),
+        "gen64_9mat": (cq.HostVolume.generate(cq.gen_default_params(6, 3, n_materials=9)), 64, (0, 0, 0)),
+    }
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_gpu_small_and_degenerate_volumes(cq, oracle, gpu_ctx, variant):
+    from tests.raysets import special_rays
+    from tests.test_oracle_pinning import _brute_rays
+
+    for name, (hv, n, lower) in _small_volumes(cq).items():
+        dv = gpu_ctx.upload(hv)
+        sp = special_rays(n)
+        sp[:, :3] += np.asarray(lower, np.float32)
+        rays = np.concatenate([_brute_rays(n, lower, 4000, 17), sp]).astype(np.float32)
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        for surf, fp in ((True, -1.0), (False, -1.0), (True, 0.1)):
+            # includes rays the reference would never finish (NaN exits); both sides cap them
+            o = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+            g = _gpu(gpu_ctx, dv, rays, surf, fp, cq.LaunchCfg(variant=variant, max_iters=1 << 16))
+            eq = records_equal(g, o)
+            assert eq.all(), _report(f"{name} surf={surf} fp={fp}", eq, rays, g, o)
+        dv.free()
+
+
+def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
+    """The benchmark workload at full size: all 2^24 incoherent rays bit-exact against the
+    oracle; both kernel variants agree; hit voxels hold the hit material (Volume::voxel)."""
+    import time
+
+    from cubiquity_amd import workloads as W
+
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    rays = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)
+    dv = gpu_ctx.upload(hv)
+    t0 = time.time()
+    g2 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=2))
+    g1 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=1))
+    print(f"gpu {time.time() - t0:.2f}s", flush=True)
+    assert records_equal(g1, g2).all()
+    t0 = time.time()
+    o = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, threads=16)
+    print(f"oracle {time.time() - t0:.2f}s", flush=True)
+    eq = records_equal(g2, o)
+    assert eq.all(), _report("1024 incoherent", eq, rays, g2, o)
+    assert ((g2["info"] >> 15) & 1).sum() == 0
+    hit = np.nonzero(g2["info"] & 1)[0]
+    assert 0.3 < len(hit) / len(rays) < 0.95
+    rng = np.random.Generator(np.random.PCG64(5))
+    for i in rng.choice(hit, 2000, replace=False):
+        x, y, z = (int(v) for v in g2["voxel"][i])
+        assert hv.voxel(x, y, z) == (int(g2["info"][i]) >> 16) & 255
+    dv.free()
+
+
+def test_gpu_deterministic(cq, gpu_ctx, dvol128, rays128):
+    a = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
+    b = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
+    assert records_equal(a, b).all()
+
+
+def test_gpu_iteration_cap_reports_anomaly(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
+    """A tiny per-ray iteration cap forced from the host: every capped ray is flagged, none hangs."""
+    rays = rays128["camera"][30720:34816]  # image centre: every ray enters the volume
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    o = oracle.intersect(vol128.nodes, sub, rays, True, -1.0, max_iters=5)
+    g = _gpu(gpu_ctx, dvol128, rays, True, -1.0, cq.LaunchCfg(max_iters=5))
+    assert ((g["info"] >> 15) & 1).sum() > 0
+    assert records_equal(g, o).all()
+
+
+def test_gpu_empty_and_single(cq, gpu_ctx, dvol128, rays128):
+    import torch
+
+    from cubiquity_amd import intersectVolume
+
+    empty = torch.zeros((0, 6), dtype=torch.float32, device="cuda")
+    assert intersectVolume(gpu_ctx, dvol128, empty).numel() == 0
+    one = _gpu(gpu_ctx, dvol128, rays128["camera"][32896:32897])
+    assert one.shape == (1,)
+
+
+def test_gpu_host_entry_point(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
+    rays = rays128["special"]
+    g = gpu_ctx.intersect_host(dvol128, rays)
+    o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays)
+    assert records_equal(g, o).all()
