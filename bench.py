@@ -83,6 +83,74 @@ def cpu_baseline(hv, rays, hits_gpu, cq):
     }
 
 
+def timed(fn, reps, warmup=1):
+    """Average milliseconds of fn() over `reps` runs, bracketed by HIP events on torch's stream."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def bench_pathtrace(cq, W, ctx, reps):
+    """Forward path tracing of the 1024^3 volume: 1280x720, 16 spp, 4 bounces (both renderers)."""
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    dv = ctx.upload(hv)
+    alb, em = cq.material_table(9, 11)
+    s = cq.PathTraceSetup(w.camera, 1280, 720, 16, 4, alb, em)
+    out = {}
+    for mode, name in ((0, "wavefront_graph"), (1, "megakernel")):
+        ms = timed(lambda: cq.pathtrace(ctx, dv, s, mode), reps)
+        out[name + "_ms"] = round(ms, 3)
+    best = min(out.values())
+    out["paths_per_frame"] = 1280 * 720 * 16
+    out["mpaths_per_s"] = round(1280 * 720 * 16 / (best * 1e-3) / 1e6, 2)
+    dv.free()
+    return out
+
+
+def bench_adjoint(cq, W, ctx, reps):
+    """Differentiable rendering (512^3, 16 materials, 512x512, 8 spp, 3 bounces): forward time,
+    adjoint (path replay) time and their ratio."""
+    import torch
+
+    w = W.DIFFRENDER_512
+    hv = cq.HostVolume.generate(w.params())
+    dv = ctx.upload(hv)
+    alb, em = cq.material_table(16, 11)
+    tgt, _ = cq.material_table(16, 12)
+    s = cq.PathTraceSetup(w.camera, 512, 512, 8, 3, alb, em)
+    target = cq.pathtrace(ctx, dv, s.with_tables(tgt, em), 1).clone()
+    fwd = {}
+    for mode in (0, 1):
+        fwd[mode] = timed(lambda: cq.pathtrace(ctx, dv, s, mode), reps)
+    img = cq.pathtrace(ctx, dv, s, 1)
+    d_image = (2 * (img - target)).contiguous()
+    loss = float(((img.double() - target.double()) ** 2).sum())
+    adj = timed(lambda: cq.pathtrace_adjoint(ctx, dv, s, d_image), reps)
+    da, de = cq.pathtrace_adjoint(ctx, dv, s, d_image)
+    torch.cuda.synchronize()
+    dv.free()
+    f = min(fwd.values())
+    return {
+        "forward_ms": round(f, 3),
+        "forward_wavefront_ms": round(fwd[0], 3),
+        "forward_megakernel_ms": round(fwd[1], 3),
+        "adjoint_ms": round(adj, 3),
+        "ratio": round(adj / f, 3),
+        "loss": loss,
+        "grad_norm": float(torch.cat([da.flatten(), de.flatten()]).double().norm()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +159,7 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pathtrace", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard by "
@@ -179,6 +248,14 @@ def main():
         "adjoint_forward_ratio": None,
         "cpu_baseline": None,
     }
+    del d_hits, d_rays
+    if not args.no_pathtrace:
+        out["pathtrace_1024"] = bench_pathtrace(cq, W, ctx, max(1, args.steps // 5))
+        log(f"path tracing: {out['pathtrace_1024']}")
+        adj = bench_adjoint(cq, W, ctx, max(2, args.steps // 4))
+        out["adjoint_forward_ratio"] = adj["ratio"]
+        out["diffrender_512"] = adj
+        log(f"adjoint: {adj}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hv, rays, hits, cq)
         out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

@@ -240,6 +240,82 @@ class Context:
             pass
 
 
+def material_table(n_materials, seed, n_emissive=1, emission=5.0):
+    """Albedo U[0.1, 0.9]^3 per material (numpy PCG64(seed)); the last `n_emissive` materials
+    emit `emission` on every channel. Rows are material ids 1..n_materials."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    albedo = rng.uniform(0.1, 0.9, (n_materials, 3)).astype(np.float32)
+    em = np.zeros((n_materials, 3), np.float32)
+    if n_emissive:
+        em[-n_emissive:] = emission
+    return albedo, em
+
+
+class PathTraceSetup:
+    """Parameters of one render (cqx_pt_params); keeps its material tables alive."""
+
+    def __init__(self, camera, width, height, spp, bounces, albedo, emission, sky=(0.8, 0.8, 1.0),
+                 bounce_offset=0.01, max_footprint=MAX_FOOTPRINT_DISABLED, seed=0):
+        self.albedo = np.ascontiguousarray(albedo, dtype=np.float32).reshape(-1, 3)
+        self.emission = np.ascontiguousarray(emission, dtype=np.float32).reshape(-1, 3)
+        if self.albedo.shape != self.emission.shape:
+            raise ValueError("albedo and emission tables differ in size")
+        p = cabi.PtParams()
+        p.width, p.height, p.spp, p.bounces = int(width), int(height), int(spp), int(bounces)
+        p.eye = (ctypes.c_double * 3)(*camera.eye)
+        p.target = (ctypes.c_double * 3)(*camera.target)
+        p.fov_y_degrees = float(camera.fov_y)
+        p.n_materials = self.albedo.shape[0]
+        p.albedo = self.albedo.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        p.emission = self.emission.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        p.sky = (ctypes.c_float * 3)(*sky)
+        p.bounce_offset = float(bounce_offset)
+        p.max_footprint = float(max_footprint)
+        p.seed = int(seed)
+        self.params = p
+        self.width, self.height, self.spp, self.bounces = p.width, p.height, p.spp, p.bounces
+        self.camera = camera
+        self.sky = tuple(sky)
+        self.bounce_offset = bounce_offset
+        self.max_footprint = max_footprint
+        self.seed = seed
+
+    def with_tables(self, albedo, emission):
+        return PathTraceSetup(self.camera, self.width, self.height, self.spp, self.bounces, albedo, emission,
+                              self.sky, self.bounce_offset, self.max_footprint, self.seed)
+
+
+def pathtrace(ctx, dvol, setup, mode=0, cfg=None, out=None, stream=None):
+    """Render (cqx_pathtrace): returns a float32 GPU tensor (height, width, 3), linear RGB."""
+    import torch
+
+    if out is None:
+        out = torch.empty((setup.height, setup.width, 3), dtype=torch.float32, device="cuda")
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    cfgp = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().cqx_pathtrace(ctx._h, dvol._h, ctypes.byref(setup.params), int(mode), cfgp,
+                              ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+    return out
+
+
+def pathtrace_adjoint(ctx, dvol, setup, d_image, cfg=None, stream=None):
+    """dLoss/dalbedo and dLoss/demission (n_materials x 3 GPU tensors) for dLoss/dimage `d_image`."""
+    import torch
+
+    d_image = d_image.contiguous()
+    m = setup.albedo.shape[0]
+    da = torch.empty((m, 3), dtype=torch.float32, device="cuda")
+    de = torch.empty((m, 3), dtype=torch.float32, device="cuda")
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    cfgp = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().cqx_pathtrace_adjoint(ctx._h, dvol._h, ctypes.byref(setup.params), cfgp,
+                                      ctypes.c_void_p(d_image.data_ptr()), ctypes.c_void_p(da.data_ptr()),
+                                      ctypes.c_void_p(de.data_ptr()), ctypes.c_void_p(stream)))
+    return da, de
+
+
 def intersectVolume(ctx, dvol, rays, computeSurfaceProperties=True, maxFootprint=MAX_FOOTPRINT_DISABLED,
                     cfg=None, stream=None, out=None):
     """Batched intersectVolume (src/library/raytracing.cpp:397-478) on device-resident rays.

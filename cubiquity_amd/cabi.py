@@ -44,6 +44,17 @@ class LaunchCfg(ctypes.Structure):
                 ("max_iters", ctypes.c_uint32)]
 
 
+class PtParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("bounces", ctypes.c_uint32), ("eye", ctypes.c_double * 3), ("target", ctypes.c_double * 3),
+                ("fov_y_degrees", ctypes.c_double), ("n_materials", ctypes.c_uint32),
+                ("albedo", ctypes.POINTER(ctypes.c_float)), ("emission", ctypes.POINTER(ctypes.c_float)),
+                ("sky", ctypes.c_float * 3), ("bounce_offset", ctypes.c_float), ("max_footprint", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+assert ctypes.sizeof(PtParams) == 128
+
 _LIB = None
 
 _P = ctypes.c_void_p
@@ -79,6 +90,8 @@ SIGNATURES = {
     "cqx_volume_info": (_ST, [_P, _P, _P, _P, _P]),
     "cqx_intersect": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_host": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P]),
+    "cqx_pathtrace": (_ST, [_P, _P, _P, _U32, _P, _P, _P]),
+    "cqx_pathtrace_adjoint": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 

@@ -1,0 +1,56 @@
+// Internal declarations shared by the HIP sources of libcqx.so (runtime.hip, pathtrace.hip).
+#pragma once
+
+#include "cqx_internal.h"
+#include "traverse_core.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+
+#define HIP_TRY(expr)                                                                                            \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess) return cqx::fail(CQX_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+namespace cqx {
+
+struct PtPlan; // path-tracer buffers and captured graph (pathtrace.hip)
+void destroy_pt_plan(PtPlan* plan);
+
+// Best-effort release paths: an error here has nowhere useful to go.
+inline void release(void* p) { (void)hipFree(p); }
+
+constexpr uint32_t kDefaultMaxIters = 1u << 22;
+
+} // namespace cqx
+
+struct cqx_ctx {
+    int device = 0;
+    int cu_count = 0;
+    int lds_per_block = 0;
+    uint64_t* d_counter = nullptr; // work counters of the persistent kernel (8 queues)
+    void* scratch = nullptr;       // staging for cqx_intersect_host
+    size_t scratch_bytes = 0;
+    hipStream_t capture_stream = nullptr; // private stream for graph capture
+    cqx::PtPlan* pt_plan = nullptr;
+};
+
+struct cqx_dvolume {
+    uint32_t* nodes = nullptr;
+    cqx_subdag* sub = nullptr;
+    uint64_t node_count = 0;
+    uint32_t root = 0;
+    cqx_subdag hsub[8];
+    int max_height = 0; // largest SubDAG start height (stack rows needed: max_height + 1)
+};
+
+namespace cqx {
+
+// Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).
+cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                            const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
+                            const cqx_launch_cfg* cfg, hipStream_t stream);
+
+} // namespace cqx

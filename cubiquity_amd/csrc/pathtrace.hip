@@ -1,0 +1,670 @@
+// Path tracer and its path-replay adjoint on the gfx950 traversal.
+//
+// Shading model (no reference counterpart: the reference's CPU and GLSL tracers use sun + sky
+// shadow rays and one bounce, src/application/commands/view/pathtracing_demo.cpp:81-190; the
+// conventions kept from them are the hit position o + d t, the bounce-ray origin offset along the
+// face normal, pathtracing_demo.cpp:139, and the light-blue background (0.8, 0.8, 1.0), :124):
+//   * per path (pixel p, sample s): PCG32 with initstate (seed << 32) + p and stream s;
+//   * camera ray through (x + u, y + v) (two draws), pinhole, y down;
+//   * at each hit: L += beta * emission[m]; if fewer than `bounces` scatterings so far, sample a
+//     cosine-weighted direction about the face normal (two draws), beta *= albedo[m];
+//   * a ray that escapes adds beta * sky; a path ends after `bounces` scatterings.
+// Lambertian BRDF albedo/pi with cosine-weighted sampling makes the throughput weight albedo.
+// sin/cos of 2 pi u use the quadrant reduction + Taylor polynomials below and sqrt is correctly
+// rounded, so the same arithmetic on the CPU reproduces every path bit for bit.
+//
+// Forward, two forms with identical per-path arithmetic:
+//   wavefront  generate -> (intersect, shade) x (bounces + 1), captured once in a hipGraph per
+//              plan and replayed; ray counts live on the device, the persistent traversal kernel
+//              reads them; surviving rays are compacted by one atomic per wave
+//   megakernel one lane per path, grid-stride
+// Both write per-path radiance; k_pt_resolve sums the samples of a pixel in sample order.
+// Adjoint (path replay): regenerates each path from its RNG (nothing is stored by the forward
+// pass), keeps the <= 8 vertices of the path in registers and sweeps them backwards,
+//   G_K = sky if escaped else 0;  G_j = E_j + A_j G_{j+1};
+//   dV/dE_j = beta_j,  dV/dA_j = beta_j G_{j+1},
+// scaled by dL/dpixel / spp; per-block partial sums in LDS, reduced over blocks in FP64.
+#include "device_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+using namespace cqxd;
+using cqx::fail;
+using cqx::release;
+
+namespace {
+
+constexpr int kMaxVertices = 8; // adjoint: bounces + 1 <= 8
+
+struct PtDevParams {
+    float eye[3];
+    float fwd[3], right[3], up[3];
+    float kx, ky, half_w, half_h;
+    float sky[3];
+    float offset;
+    float max_footprint;
+    uint32_t width, height, spp, bounces;
+    uint32_t pad;
+    uint64_t seed;
+    float albedo[256 * 3];   // by material id
+    float emission[256 * 3]; // by material id
+};
+
+// ---- PCG32 (O'Neill 2014) ---------------------------------------------------------------------
+struct Pcg {
+    uint64_t state, inc;
+};
+
+__device__ __forceinline__ uint32_t pcg_next(Pcg& r)
+{
+    const uint64_t old = r.state;
+    r.state = old * 6364136223846793005ULL + r.inc;
+    const uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+    const uint32_t rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+
+__device__ __forceinline__ Pcg pcg_seed(uint64_t initstate, uint64_t initseq)
+{
+    Pcg r;
+    r.state = 0;
+    r.inc = (initseq << 1u) | 1u;
+    pcg_next(r);
+    r.state += initstate;
+    pcg_next(r);
+    return r;
+}
+
+// uniform in [0, 1) with 24 random bits (exact conversion and scaling)
+__device__ __forceinline__ float pcg_uniform(Pcg& r) { return float(pcg_next(r) >> 8) * (1.0f / 16777216.0f); }
+
+// cos and sin of 2 pi u for u in [0, 1): quadrant q = floor(4u), theta = (4u - q) pi/2 in [0, pi/2),
+// Taylor polynomials of degree 11 (sin) and 12 (cos), then the quadrant rotation.
+__device__ __forceinline__ void sincos_2pi(float u, float& c, float& s)
+{
+    const float u4 = u * 4.0f;
+    const float q = floorf(u4);
+    const float th = (u4 - q) * 1.57079632679489662f;
+    const float t2 = th * th;
+    const float sp = th * (1.0f + t2 * (-1.66666667e-1f + t2 * (8.33333333e-3f + t2 * (-1.98412698e-4f +
+                     t2 * (2.75573192e-6f + t2 * -2.50521084e-8f)))));
+    const float cp = 1.0f + t2 * (-0.5f + t2 * (4.16666667e-2f + t2 * (-1.38888889e-3f + t2 * (2.48015873e-5f +
+                     t2 * (-2.75573192e-7f + t2 * 2.08767570e-9f)))));
+    const int qi = int(q);
+    if (qi == 0) { c = cp; s = sp; }
+    else if (qi == 1) { c = -sp; s = cp; }
+    else if (qi == 2) { c = -cp; s = -sp; }
+    else { c = sp; s = -cp; }
+}
+
+__device__ __forceinline__ void camera_ray(const PtDevParams& d, uint32_t x, uint32_t y, float jx, float jy,
+                                           float* o, float* dir)
+{
+    const float cx = ((float(x) + jx) - d.half_w) * d.kx;
+    const float cy = ((float(y) + jy) - d.half_h) * d.ky;
+    float v[3];
+    for (int a = 0; a < 3; a++) v[a] = (d.fwd[a] + d.right[a] * cx) - d.up[a] * cy;
+    const float len = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+    for (int a = 0; a < 3; a++) {
+        o[a] = d.eye[a];
+        dir[a] = v[a] / len;
+    }
+}
+
+// Face normal (components -1, -0, +0, +1) from a hit record's normal code.
+__device__ __forceinline__ void decode_normal(uint32_t info, float* n)
+{
+    for (int a = 0; a < 3; a++) {
+        const uint32_t code = (info >> (9 + 2 * a)) & 3u;
+        const float mag = (code & 1u) ? 1.0f : 0.0f;
+        n[a] = (code & 2u) ? -mag : mag;
+    }
+}
+
+// Cosine-weighted direction about the face normal n and the next ray's origin
+// (pos + n * offset). Orthonormal basis after Duff et al., "Building an Orthonormal Basis,
+// Revisited", JCGT 6(1), 2017. Returns false for a degenerate (zero) normal.
+__device__ __forceinline__ bool scatter(const PtDevParams& d, Pcg& rng, const float* pos, const float* n, float* o,
+                                        float* dir)
+{
+    const float nl = sqrtf((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]);
+    if (!(nl > 0.0f)) return false;
+    const float nx = n[0] / nl, ny = n[1] / nl, nz = n[2] / nl;
+    const float u1 = pcg_uniform(rng);
+    const float u2 = pcg_uniform(rng);
+    const float r = sqrtf(u1);
+    float c, s;
+    sincos_2pi(u2, c, s);
+    const float lx = r * c, ly = r * s, lz = sqrtf(1.0f - u1);
+    const float sign = copysignf(1.0f, nz);
+    const float a = -1.0f / (sign + nz);
+    const float b = nx * ny * a;
+    const float t0 = 1.0f + sign * nx * nx * a, t1 = sign * b, t2 = -sign * nx;
+    const float u0 = b, uu1 = sign + ny * ny * a, u2b = -ny;
+    dir[0] = (t0 * lx + u0 * ly) + nx * lz;
+    dir[1] = (t1 * lx + uu1 * ly) + ny * lz;
+    dir[2] = (t2 * lx + u2b * ly) + nz * lz;
+    for (int k = 0; k < 3; k++) o[k] = pos[k] + n[k] * d.offset;
+    return true;
+}
+
+__device__ __forceinline__ void path_start(const PtDevParams& d, uint64_t p, Pcg& rng, float* o, float* dir,
+                                           uint32_t& pixel)
+{
+    pixel = uint32_t(p / d.spp);
+    const uint32_t s = uint32_t(p % d.spp);
+    rng = pcg_seed((d.seed << 32) + uint64_t(pixel), uint64_t(s));
+    const float jx = pcg_uniform(rng);
+    const float jy = pcg_uniform(rng);
+    camera_ray(d, pixel % d.width, pixel / d.width, jx, jy, o, dir);
+}
+
+// ---- wavefront kernels -----------------------------------------------------------------------
+
+__global__ void k_pt_generate(const PtDevParams* __restrict__ dp, uint64_t P, cqx_ray* __restrict__ rays,
+                              uint32_t* __restrict__ paths, float* __restrict__ beta, float* __restrict__ L,
+                              uint64_t* __restrict__ rng_state, uint32_t* __restrict__ count)
+{
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p == 0) *count = uint32_t(P);
+    if (p >= P) return;
+    const PtDevParams& d = *dp;
+    Pcg rng;
+    float o[3], dir[3];
+    uint32_t pixel;
+    path_start(d, p, rng, o, dir, pixel);
+    cqx_ray r;
+    for (int a = 0; a < 3; a++) {
+        r.o[a] = o[a];
+        r.d[a] = dir[a];
+        beta[a * P + p] = 1.0f;
+        L[a * P + p] = 0.0f;
+    }
+    rays[p] = r;
+    paths[p] = uint32_t(p);
+    rng_state[p] = rng.state;
+}
+
+__global__ void k_set_u32(uint32_t* __restrict__ x, uint32_t v)
+{
+    if (threadIdx.x == 0) *x = v;
+}
+
+__global__ void __launch_bounds__(256) k_pt_shade(const PtDevParams* __restrict__ dp, uint64_t P, uint32_t depth,
+                                                  const uint32_t* __restrict__ count_cur,
+                                                  const cqx_ray* __restrict__ rays_cur,
+                                                  const uint32_t* __restrict__ paths_cur,
+                                                  const cqx_hit* __restrict__ hits, float* __restrict__ beta,
+                                                  float* __restrict__ L, uint64_t* __restrict__ rng_state,
+                                                  cqx_ray* __restrict__ rays_next, uint32_t* __restrict__ paths_next,
+                                                  uint32_t* __restrict__ count_next)
+{
+    const PtDevParams& d = *dp;
+    const uint32_t n = *count_cur;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cont = false;
+    uint32_t p = 0;
+    cqx_ray next;
+    if (i < n) {
+        p = paths_cur[i];
+        const cqx_hit h = hits[i];
+        float b[3] = { beta[p], beta[P + p], beta[2 * P + p] };
+        if (h.info & CQX_HIT_ANOMALY) {
+            // traversal gave up on this ray: the path ends without a contribution
+        } else if (!(h.info & CQX_HIT_FLAG)) {
+            for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.sky[a];
+        } else {
+            const uint32_t m = (h.info >> 16) & 0xffu;
+            for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.emission[m * 3 + a];
+            if (depth < d.bounces) {
+                const cqx_ray r = rays_cur[i];
+                float pos[3], nrm[3];
+                for (int a = 0; a < 3; a++) pos[a] = r.o[a] + r.d[a] * h.t;
+                decode_normal(h.info, nrm);
+                Pcg rng;
+                rng.state = rng_state[p];
+                rng.inc = ((uint64_t(p % d.spp)) << 1u) | 1u;
+                if (scatter(d, rng, pos, nrm, next.o, next.d)) {
+                    rng_state[p] = rng.state;
+                    for (int a = 0; a < 3; a++) beta[a * P + p] = b[a] * d.albedo[m * 3 + a];
+                    cont = true;
+                }
+            }
+        }
+    }
+    // compact the surviving rays: one atomic per wave, ballot prefix for the slots
+    const uint64_t mask = __ballot(cont);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (mask) {
+        if (lane == uint32_t(__ffsll((long long)mask) - 1)) base = atomicAdd(count_next, uint32_t(__popcll(mask)));
+        base = __shfl(base, __ffsll((long long)mask) - 1);
+    }
+    if (cont) {
+        const uint32_t j = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        rays_next[j] = next;
+        paths_next[j] = p;
+    }
+}
+
+__global__ void k_pt_resolve(const PtDevParams* __restrict__ dp, uint64_t P, const float* __restrict__ L,
+                             float* __restrict__ image)
+{
+    const PtDevParams& d = *dp;
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.width * d.height) return;
+    for (int a = 0; a < 3; a++) {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < d.spp; s++) sum += L[a * P + uint64_t(q) * d.spp + s];
+        image[uint64_t(q) * 3 + a] = sum / float(d.spp);
+    }
+}
+
+// ---- megakernel forward ------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __restrict__ dp, TraceParams T, uint64_t P,
+                                                       float* __restrict__ L)
+{
+    extern __shared__ uint32_t lds_stack[];
+    const PtDevParams& d = *dp;
+    const uint64_t stride_all = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < P; p += stride_all) {
+        Pcg rng;
+        float o[3], dir[3];
+        uint32_t pixel;
+        path_start(d, p, rng, o, dir, pixel);
+        float b[3] = { 1.0f, 1.0f, 1.0f }, l[3] = { 0.0f, 0.0f, 0.0f };
+        for (uint32_t depth = 0;; depth++) {
+            cqx_hit h;
+            trace_ray(T, o[0], o[1], o[2], dir[0], dir[1], dir[2], lds_stack + threadIdx.x, blockDim.x, h);
+            if (h.info & CQX_HIT_ANOMALY) break;
+            if (!(h.info & CQX_HIT_FLAG)) {
+                for (int a = 0; a < 3; a++) l[a] += b[a] * d.sky[a];
+                break;
+            }
+            const uint32_t m = (h.info >> 16) & 0xffu;
+            for (int a = 0; a < 3; a++) l[a] += b[a] * d.emission[m * 3 + a];
+            if (depth >= d.bounces) break;
+            float pos[3], nrm[3];
+            for (int a = 0; a < 3; a++) pos[a] = o[a] + dir[a] * h.t;
+            decode_normal(h.info, nrm);
+            if (!scatter(d, rng, pos, nrm, o, dir)) break;
+            for (int a = 0; a < 3; a++) b[a] = b[a] * d.albedo[m * 3 + a];
+        }
+        for (int a = 0; a < 3; a++) L[a * P + p] = l[a];
+    }
+}
+
+// ---- adjoint (path replay) -----------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restrict__ dp, TraceParams T, uint64_t P,
+                                                    const float* __restrict__ dimage, uint32_t M,
+                                                    float* __restrict__ partials)
+{
+    extern __shared__ uint32_t lds_stack[];
+    __shared__ float g[256 * 6]; // [material][dA rgb, dE rgb]
+    for (uint32_t k = threadIdx.x; k < 256 * 6; k += blockDim.x) g[k] = 0.0f;
+    __syncthreads();
+    const PtDevParams& d = *dp;
+    const float inv_spp = 1.0f / float(d.spp);
+    const uint64_t stride_all = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < P; p += stride_all) {
+        Pcg rng;
+        float o[3], dir[3];
+        uint32_t pixel;
+        path_start(d, p, rng, o, dir, pixel);
+        float delta[3];
+        for (int a = 0; a < 3; a++) delta[a] = dimage[uint64_t(pixel) * 3 + a] * inv_spp;
+        float b[3] = { 1.0f, 1.0f, 1.0f };
+        uint32_t vm[kMaxVertices];
+        float vb[kMaxVertices][3];
+        int K = 0;
+        bool escaped = false;
+        for (uint32_t depth = 0; depth < uint32_t(kMaxVertices); depth++) {
+            cqx_hit h;
+            trace_ray(T, o[0], o[1], o[2], dir[0], dir[1], dir[2], lds_stack + threadIdx.x, blockDim.x, h);
+            if (h.info & CQX_HIT_ANOMALY) break;
+            if (!(h.info & CQX_HIT_FLAG)) {
+                escaped = true;
+                break;
+            }
+            const uint32_t m = (h.info >> 16) & 0xffu;
+#pragma unroll
+            for (int j = 0; j < kMaxVertices; j++)
+                if (j == K) {
+                    vm[j] = m;
+                    vb[j][0] = b[0];
+                    vb[j][1] = b[1];
+                    vb[j][2] = b[2];
+                }
+            K++;
+            if (depth >= d.bounces) break;
+            float pos[3], nrm[3];
+            for (int a = 0; a < 3; a++) pos[a] = o[a] + dir[a] * h.t;
+            decode_normal(h.info, nrm);
+            if (!scatter(d, rng, pos, nrm, o, dir)) break;
+            for (int a = 0; a < 3; a++) b[a] = b[a] * d.albedo[m * 3 + a];
+        }
+        float G[3];
+        for (int a = 0; a < 3; a++) G[a] = escaped ? d.sky[a] : 0.0f;
+#pragma unroll
+        for (int j = kMaxVertices - 1; j >= 0; j--) {
+            if (j < K) {
+                const uint32_t m = vm[j];
+                for (int a = 0; a < 3; a++) {
+                    const float w = delta[a] * vb[j][a];
+                    atomicAdd(&g[m * 6 + a], w * G[a]);
+                    atomicAdd(&g[m * 6 + 3 + a], w);
+                    G[a] = d.emission[m * 3 + a] + d.albedo[m * 3 + a] * G[a];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < M * 6; k += blockDim.x)
+        partials[uint64_t(blockIdx.x) * M * 6 + k] = g[6 + k]; // material ids 1..M
+}
+
+__global__ void k_pt_reduce(const float* __restrict__ partials, uint32_t blocks, uint32_t M, float* __restrict__ d_albedo,
+                            float* __restrict__ d_emission)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; // (material - 1) * 6 + component
+    if (k >= M * 6) return;
+    double sum = 0.0;
+    for (uint32_t b = 0; b < blocks; b++) sum += double(partials[uint64_t(b) * M * 6 + k]);
+    const uint32_t m = k / 6, c = k % 6;
+    if (c < 3) d_albedo[m * 3 + c] = float(sum);
+    else d_emission[m * 3 + (c - 3)] = float(sum);
+}
+
+} // namespace
+
+namespace cqx {
+
+struct PtPlan {
+    const cqx_dvolume* vol = nullptr;
+    uint32_t width = 0, height = 0, spp = 0, bounces = 0;
+    float max_footprint = -1.0f;
+    cqx_launch_cfg cfg{};
+    uint64_t P = 0;
+    PtDevParams* dparams = nullptr;
+    cqx_ray* rays[2] = { nullptr, nullptr };
+    uint32_t* paths[2] = { nullptr, nullptr };
+    cqx_hit* hits = nullptr;
+    float* beta = nullptr;
+    float* L = nullptr;
+    uint64_t* rng = nullptr;
+    uint32_t* counts = nullptr;
+    float* partials = nullptr;
+    size_t partial_floats = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+void destroy_pt_plan(PtPlan* p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    release(p->dparams);
+    for (int k = 0; k < 2; k++) {
+        release(p->rays[k]);
+        release(p->paths[k]);
+    }
+    release(p->hits);
+    release(p->beta);
+    release(p->L);
+    release(p->rng);
+    release(p->counts);
+    release(p->partials);
+    delete p;
+}
+
+} // namespace cqx
+
+namespace {
+
+using cqx::PtPlan;
+
+cqx_status fill_params(const cqx_pt_params* pp, PtDevParams& d)
+{
+    std::memset(&d, 0, sizeof(d));
+    if (!pp->width || !pp->height || !pp->spp) return fail(CQX_ERR_INVALID_ARG, "image size and spp must be positive");
+    if (pp->bounces > 64) return fail(CQX_ERR_LIMIT, "at most 64 bounces");
+    if (pp->n_materials > 255) return fail(CQX_ERR_INVALID_ARG, "at most 255 materials");
+    if (pp->n_materials && (!pp->albedo || !pp->emission)) return fail(CQX_ERR_INVALID_ARG, "null material table");
+    double f[3] = { pp->target[0] - pp->eye[0], pp->target[1] - pp->eye[1], pp->target[2] - pp->eye[2] };
+    const double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (!(fl > 0)) return fail(CQX_ERR_INVALID_ARG, "eye equals target");
+    for (double& c : f) c /= fl;
+    double r[3] = { -f[2], 0.0, f[0] };
+    const double rl = std::sqrt(r[0] * r[0] + r[2] * r[2]);
+    if (!(rl > 0)) return fail(CQX_ERR_INVALID_ARG, "view direction parallel to +y");
+    for (double& c : r) c /= rl;
+    const double u[3] = { r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0] };
+    const double scale = 2.0 * std::tan(pp->fov_y_degrees * 3.14159265358979323846 / 360.0);
+    const double aspect = double(pp->width) / double(pp->height);
+    for (int a = 0; a < 3; a++) {
+        d.eye[a] = float(pp->eye[a]);
+        d.fwd[a] = float(f[a]);
+        d.right[a] = float(r[a]);
+        d.up[a] = float(u[a]);
+        d.sky[a] = pp->sky[a];
+    }
+    d.kx = float(aspect * scale / pp->width);
+    d.ky = float(scale / pp->height);
+    d.half_w = 0.5f * float(pp->width);
+    d.half_h = 0.5f * float(pp->height);
+    d.offset = pp->bounce_offset;
+    d.max_footprint = pp->max_footprint;
+    d.width = pp->width;
+    d.height = pp->height;
+    d.spp = pp->spp;
+    d.bounces = pp->bounces;
+    d.seed = pp->seed;
+    for (uint32_t m = 1; m <= pp->n_materials; m++)
+        for (int a = 0; a < 3; a++) {
+            d.albedo[m * 3 + a] = pp->albedo[(m - 1) * 3 + a];
+            d.emission[m * 3 + a] = pp->emission[(m - 1) * 3 + a];
+        }
+    return CQX_OK;
+}
+
+template <typename T>
+cqx_status alloc(T** p, size_t count)
+{
+    if (hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess)
+        return fail(CQX_ERR_OOM, "device allocation of %zu bytes failed", sizeof(T) * count);
+    return CQX_OK;
+}
+
+// Returns the context's plan for this volume and image shape, (re)building it when anything
+// differs. The wavefront buffers and graph are added on first use.
+cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* pp, const cqx_launch_cfg* cfg,
+                    PtPlan** out)
+{
+    cqx_launch_cfg c{};
+    if (cfg) c = *cfg;
+    PtPlan* p = ctx->pt_plan;
+    if (p && p->vol == vol && p->width == pp->width && p->height == pp->height && p->spp == pp->spp &&
+        p->bounces == pp->bounces && p->max_footprint == pp->max_footprint &&
+        std::memcmp(&p->cfg, &c, sizeof(c)) == 0) {
+        *out = p;
+        return CQX_OK;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    cqx::destroy_pt_plan(p);
+    ctx->pt_plan = nullptr;
+    p = new (std::nothrow) PtPlan;
+    if (!p) return fail(CQX_ERR_OOM, "out of host memory");
+    p->vol = vol;
+    p->width = pp->width;
+    p->height = pp->height;
+    p->spp = pp->spp;
+    p->bounces = pp->bounces;
+    p->max_footprint = pp->max_footprint;
+    p->cfg = c;
+    p->P = uint64_t(pp->width) * pp->height * pp->spp;
+    cqx_status st = alloc(&p->dparams, 1);
+    if (st == CQX_OK) st = alloc(&p->L, 3 * p->P);
+    if (st != CQX_OK) {
+        cqx::destroy_pt_plan(p);
+        return st;
+    }
+    ctx->pt_plan = p;
+    *out = p;
+    return CQX_OK;
+}
+
+// Allocates the wavefront buffers and captures generate -> (reset next count, intersect, shade) x
+// (bounces + 1) once into a graph.
+cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
+{
+    if (p->exec) return CQX_OK;
+    const uint64_t P = p->P;
+    cqx_status st = CQX_OK;
+    for (int k = 0; k < 2 && st == CQX_OK; k++) {
+        st = alloc(&p->rays[k], P);
+        if (st == CQX_OK) st = alloc(&p->paths[k], P);
+    }
+    if (st == CQX_OK) st = alloc(&p->hits, P);
+    if (st == CQX_OK) st = alloc(&p->beta, 3 * P);
+    if (st == CQX_OK) st = alloc(&p->rng, P);
+    if (st == CQX_OK) st = alloc(&p->counts, 64);
+    if (st != CQX_OK) return st;
+    hipStream_t s = ctx->capture_stream;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const uint32_t gb = uint32_t((P + 255) / 256);
+    hipLaunchKernelGGL(k_pt_generate, dim3(gb), dim3(256), 0, s, p->dparams, P, p->rays[0], p->paths[0], p->beta, p->L,
+                       p->rng, p->counts + 0);
+    cqx_status cst = CQX_OK;
+    for (uint32_t depth = 0; depth <= p->bounces && cst == CQX_OK; depth++) {
+        const int cur = depth & 1, nxt = cur ^ 1;
+        uint32_t* cnt_cur = p->counts + cur * 16;
+        uint32_t* cnt_next = p->counts + nxt * 16;
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, cnt_next, 0u);
+        cst = cqx::launch_intersect(ctx, p->vol, p->rays[cur], P, cnt_cur, CQX_SURFACE_PROPERTIES, p->max_footprint,
+                                    p->hits, &p->cfg, s);
+        hipLaunchKernelGGL(k_pt_shade, dim3(gb), dim3(256), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
+                           p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    p->graph = g;
+    if (cst != CQX_OK) return cst;
+    if (e != hipSuccess || !g) return fail(CQX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return CQX_OK;
+}
+
+cqx_status check(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* pp)
+{
+    if (!ctx || !vol || !pp) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint64_t P = uint64_t(pp->width) * pp->height * pp->spp;
+    if (P == 0) return fail(CQX_ERR_INVALID_ARG, "empty image");
+    if (P >= (uint64_t(1) << 32)) return fail(CQX_ERR_LIMIT, "at most 2^32 - 1 paths per call");
+    return CQX_OK;
+}
+
+TraceParams trace_params(const cqx_dvolume* vol, const cqx_pt_params* pp, const cqx_launch_cfg* cfg)
+{
+    TraceParams T;
+    T.nodes = vol->nodes;
+    T.sub = vol->sub;
+    T.surf = 1;
+    T.max_footprint = pp->max_footprint;
+    T.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters;
+    return T;
+}
+
+} // namespace
+
+extern "C" {
+
+cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* pp, uint32_t mode,
+                         const cqx_launch_cfg* cfg, float* image, void* stream_ptr)
+{
+    cqx_status st = check(ctx, vol, pp);
+    if (st != CQX_OK) return st;
+    if (!image) return fail(CQX_ERR_INVALID_ARG, "null image");
+    if (mode > 1) return fail(CQX_ERR_INVALID_ARG, "mode must be 0 (wavefront) or 1 (megakernel)");
+    PtDevParams host;
+    st = fill_params(pp, host);
+    if (st != CQX_OK) return st;
+    PtPlan* p = nullptr;
+    st = get_plan(ctx, vol, pp, cfg, &p);
+    if (st != CQX_OK) return st;
+    if (mode == 0) {
+        st = ensure_wavefront(ctx, p);
+        if (st != CQX_OK) return st;
+    }
+    hipStream_t s = hipStream_t(stream_ptr);
+    HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
+    if (mode == 0) {
+        HIP_TRY(hipGraphLaunch(p->exec, s));
+    } else {
+        const uint32_t block = 256;
+        const size_t lds = size_t(vol->max_height + 1) * block * sizeof(uint32_t);
+        if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS", lds);
+        const uint32_t waves = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
+        const uint64_t want = (uint64_t(ctx->cu_count) * waves * 64 + block - 1) / block;
+        const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, (p->P + block - 1) / block)));
+        hipLaunchKernelGGL(k_pt_megakernel, dim3(blocks), dim3(block), lds, s, p->dparams, trace_params(vol, pp, cfg),
+                           p->P, p->L);
+    }
+    const uint32_t pixels = pp->width * pp->height;
+    hipLaunchKernelGGL(k_pt_resolve, dim3((pixels + 255) / 256), dim3(256), 0, s, p->dparams, p->P, p->L, image);
+    HIP_TRY(hipGetLastError());
+    // the plan's parameter block is read by the kernels just queued; keep `host` alive until then
+    HIP_TRY(hipStreamSynchronize(s));
+    return CQX_OK;
+}
+
+cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* pp,
+                                 const cqx_launch_cfg* cfg, const float* d_image, float* d_albedo, float* d_emission,
+                                 void* stream_ptr)
+{
+    cqx_status st = check(ctx, vol, pp);
+    if (st != CQX_OK) return st;
+    if (!d_image || !d_albedo || !d_emission) return fail(CQX_ERR_INVALID_ARG, "null buffer");
+    if (pp->bounces + 1 > uint32_t(kMaxVertices))
+        return fail(CQX_ERR_LIMIT, "the adjoint supports at most %d bounces", kMaxVertices - 1);
+    if (pp->n_materials == 0) return fail(CQX_ERR_INVALID_ARG, "no materials to differentiate");
+    PtDevParams host;
+    st = fill_params(pp, host);
+    if (st != CQX_OK) return st;
+    PtPlan* p = nullptr;
+    st = get_plan(ctx, vol, pp, cfg, &p);
+    if (st != CQX_OK) return st;
+    const uint32_t block = 256;
+    const size_t lds = size_t(vol->max_height + 1) * block * sizeof(uint32_t);
+    if (lds + 256 * 6 * sizeof(float) > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "not enough LDS");
+    const uint32_t waves = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
+    const uint64_t want = (uint64_t(ctx->cu_count) * waves * 64 + block - 1) / block;
+    const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, (p->P + block - 1) / block)));
+    const size_t need = size_t(blocks) * pp->n_materials * 6;
+    if (need > p->partial_floats) {
+        HIP_TRY(hipDeviceSynchronize());
+        release(p->partials);
+        p->partials = nullptr;
+        p->partial_floats = 0;
+        st = alloc(&p->partials, need);
+        if (st != CQX_OK) return st;
+        p->partial_floats = need;
+    }
+    hipStream_t s = hipStream_t(stream_ptr);
+    HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), lds, s, p->dparams, trace_params(vol, pp, cfg), p->P,
+                       d_image, pp->n_materials, p->partials);
+    hipLaunchKernelGGL(k_pt_reduce, dim3((pp->n_materials * 6 + 255) / 256), dim3(256), 0, s, p->partials, blocks,
+                       pp->n_materials, d_albedo, d_emission);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return CQX_OK;
+}
+
+} // extern "C"

@@ -7,56 +7,27 @@
 //                           the wave claims a contiguous block of new rays from a global counter
 //                           with one atomic and hands them out by ballot prefix (variant 2)
 // Both keep the traversal stack in LDS (one column per lane, one row per node height).
-#include "cqx_internal.h"
-#include "traverse_core.h"
-
-#include <hip/hip_runtime.h>
+#include "device_internal.h"
 
 #include <algorithm>
 #include <cstring>
-#include <mutex>
 #include <new>
 
 using namespace cqxd;
-
-struct cqx_ctx {
-    int device = 0;
-    int cu_count = 0;
-    int lds_per_block = 0;
-    uint64_t* d_counter = nullptr; // work counter of the persistent kernel
-    void* scratch = nullptr;       // staging for cqx_intersect_host
-    size_t scratch_bytes = 0;
-};
-
-struct cqx_dvolume {
-    uint32_t* nodes = nullptr;
-    cqx_subdag* sub = nullptr;
-    uint64_t node_count = 0;
-    uint32_t root = 0;
-    cqx_subdag hsub[8];
-    int max_height = 0; // largest SubDAG start height (stack rows needed: max_height + 1)
-};
+using cqx::fail;
+using cqx::release;
 
 namespace {
 
-using cqx::fail;
-
-#define HIP_TRY(expr)                                                                                  \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess) return fail(CQX_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-constexpr uint32_t kDefaultMaxIters = 1u << 22;
-
-// Best-effort release paths: an error here has nowhere useful to go.
-inline void release(void* p) { (void)hipFree(p); }
-
 // ---- variant 1: one ray per lane --------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_intersect_lane(TraceParams P, const cqx_ray* __restrict__ rays, uint64_t n,
-                                                        cqx_hit* __restrict__ hits)
+                                                        const uint32_t* __restrict__ n_dev, cqx_hit* __restrict__ hits)
 {
     extern __shared__ uint32_t lds_stack[];
+    if (n_dev) {
+        const uint64_t nd = *n_dev;
+        n = nd < n ? nd : n;
+    }
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = reinterpret_cast<const float*>(rays + i);
@@ -82,11 +53,17 @@ __global__ void k_reset_counters(unsigned long long* counters)
 }
 
 __global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, const cqx_ray* __restrict__ rays,
-                                                              uint64_t n, cqx_hit* __restrict__ hits,
+                                                              uint64_t n, const uint32_t* __restrict__ n_dev,
+                                                              cqx_hit* __restrict__ hits,
                                                               unsigned long long* __restrict__ counters,
                                                               uint32_t regroup, uint32_t refill)
 {
     extern __shared__ uint32_t lds_stack[];
+    if (n_dev) {
+        // ray count produced on the device by an earlier kernel (path-tracer bounce loop)
+        const uint64_t nd = *n_dev;
+        n = nd < n ? nd : n;
+    }
     uint32_t* stack = lds_stack + threadIdx.x;
     const uint32_t stride = blockDim.x;
     const uint32_t lane = threadIdx.x & 63u;
@@ -144,6 +121,53 @@ cqx_status check_ctx(cqx_ctx* ctx)
 
 } // namespace
 
+namespace cqx {
+
+cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                            const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
+                            const cqx_launch_cfg* cfg, hipStream_t stream)
+{
+    TraceParams P;
+    P.nodes = vol->nodes;
+    P.sub = vol->sub;
+    P.surf = (flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
+    P.max_footprint = max_footprint;
+    P.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : kDefaultMaxIters;
+
+    const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
+    if (block % 64 != 0 || block == 0 || block > 256)
+        return fail(CQX_ERR_INVALID_ARG, "block %u must be a multiple of 64, <= 256", block);
+    const uint32_t rows = uint32_t(vol->max_height) + 1;
+    const size_t lds = size_t(rows) * block * sizeof(uint32_t);
+    if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
+
+    const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
+    if (variant == 1) {
+        const uint64_t blocks = (n + block - 1) / block;
+        if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
+        hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
+    } else if (variant == 2) {
+        const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
+        const uint32_t regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
+        const uint32_t refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
+        if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
+            return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
+        const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
+        const uint64_t need_blocks = (n + block - 1) / block;
+        const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
+        hipLaunchKernelGGL(k_intersect_persistent, dim3(blocks), dim3(block), lds, stream, P, rays, n, n_dev, hits,
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
+    } else {
+        return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
+    }
+    HIP_TRY(hipGetLastError());
+    return CQX_OK;
+}
+
+} // namespace cqx
+
 extern "C" {
 
 cqx_status cqx_create(int device, cqx_ctx** out)
@@ -165,6 +189,11 @@ cqx_status cqx_create(int device, cqx_ctx** out)
         delete c;
         return fail(CQX_ERR_OOM, "device allocation of the work counters failed");
     }
+    if (hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        release(c->d_counter);
+        delete c;
+        return fail(CQX_ERR_HIP, "stream creation failed");
+    }
     *out = c;
     return CQX_OK;
 }
@@ -173,8 +202,11 @@ cqx_status cqx_destroy(cqx_ctx* ctx)
 {
     if (!ctx) return CQX_OK;
     (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    cqx::destroy_pt_plan(ctx->pt_plan);
     release(ctx->d_counter);
     release(ctx->scratch);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
     return CQX_OK;
 }
@@ -228,6 +260,11 @@ cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol)
     if (!vol) return CQX_OK;
     cqx_status st = check_ctx(ctx);
     if (st != CQX_OK) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    if (ctx->pt_plan) {
+        cqx::destroy_pt_plan(ctx->pt_plan); // a captured graph may reference this volume
+        ctx->pt_plan = nullptr;
+    }
     release(vol->nodes);
     release(vol->sub);
     delete vol;
@@ -255,43 +292,7 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
     if (n == 0) return CQX_OK;
     if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
     if (n > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "%llu rays exceed the 2^38 limit", (unsigned long long)n);
-
-    TraceParams P;
-    P.nodes = vol->nodes;
-    P.sub = vol->sub;
-    P.surf = (flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
-    P.max_footprint = max_footprint;
-    P.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : kDefaultMaxIters;
-
-    const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
-    if (block % 64 != 0 || block > 256) return fail(CQX_ERR_INVALID_ARG, "block %u must be a multiple of 64, <= 256", block);
-    const uint32_t rows = uint32_t(vol->max_height) + 1;
-    const size_t lds = size_t(rows) * block * sizeof(uint32_t);
-    if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
-
-    const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
-    if (variant == 1) {
-        const uint64_t blocks = (n + block - 1) / block;
-        hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, hipStream_t(stream), P, rays, n,
-                           hits);
-    } else if (variant == 2) {
-        const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
-        const uint32_t regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
-        const uint32_t refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
-        if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
-            return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
-        const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
-        const uint64_t need_blocks = (n + block - 1) / block;
-        const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
-        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, hipStream_t(stream),
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
-        hipLaunchKernelGGL(k_intersect_persistent, dim3(blocks), dim3(block), lds, hipStream_t(stream), P, rays, n, hits,
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
-    } else {
-        return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
-    }
-    HIP_TRY(hipGetLastError());
-    return CQX_OK;
+    return cqx::launch_intersect(ctx, vol, rays, n, nullptr, flags, max_footprint, hits, cfg, hipStream_t(stream));
 }
 
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
@@ -303,6 +304,7 @@ cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ra
     if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
     const size_t need = size_t(n) * (sizeof(cqx_ray) + sizeof(cqx_hit));
     if (need > ctx->scratch_bytes) {
+        HIP_TRY(hipDeviceSynchronize());
         release(ctx->scratch);
         ctx->scratch = nullptr;
         ctx->scratch_bytes = 0;

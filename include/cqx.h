@@ -213,6 +213,44 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                               uint32_t flags, float max_footprint, cqx_hit* hits);
 
+/* ---- path tracer ------------------------------------------------------------------------------
+ * Replaces the reference's per-pixel CPU tracer loop (PathtracingDemo::raytrace / traceSingleRay,
+ * src/application/commands/view/pathtracing_demo.cpp:150-229) and the GLSL tracer
+ * (src/application/commands/view/glsl/pathtracing.frag:723-819) with a batched GPU renderer.
+ * Per path (pixel p, sample s): PCG32 with initstate (seed << 32) + p and stream s; the camera
+ * ray goes through a jittered point of the pixel; at each hit the path adds beta * emission and,
+ * while fewer than `bounces` scatterings were made, continues in a cosine-weighted direction about
+ * the face normal from position + normal * bounce_offset with beta *= albedo; escaping rays add
+ * beta * sky. The image is linear RGB (height x width x 3 floats), each pixel the mean of its
+ * samples. Material ids 1..n_materials index rows 0..n_materials-1 of the tables; others are
+ * black. */
+typedef struct {
+    uint32_t width, height, spp, bounces;
+    double eye[3], target[3], fov_y_degrees;
+    uint32_t n_materials;
+    const float* albedo;   /* host, n_materials x 3 */
+    const float* emission; /* host, n_materials x 3 */
+    float sky[3];          /* radiance of escaping rays (the reference background: 0.8 0.8 1.0) */
+    float bounce_offset;   /* origin offset along the face normal (reference: 0.01) */
+    float max_footprint;   /* LOD (maxFootprint); CQX_MAX_FOOTPRINT_DISABLED for full detail */
+    uint64_t seed;
+} cqx_pt_params;
+
+/* mode 0: wavefront (traversal kernel per bounce, captured in a hipGraph, replayed per call);
+ * mode 1: megakernel (one lane per path). Both give the same image bit for bit. `image` is a
+ * device pointer; synchronizes `stream` before returning. */
+cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params, uint32_t mode,
+                         const cqx_launch_cfg* cfg, float* image, void* stream);
+
+/* Gradient of a scalar image loss with respect to the material tables, by path replay: every
+ * path of cqx_pathtrace is regenerated from its RNG (nothing is stored by the forward pass),
+ * geometry and traversal are discrete and exact. d_image = dLoss/dimage (device, height x width
+ * x 3); d_albedo, d_emission (device, n_materials x 3) receive dLoss/dalbedo and
+ * dLoss/demission. At most 7 bounces. Synchronizes `stream` before returning. */
+cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params,
+                                 const cqx_launch_cfg* cfg, const float* d_image, float* d_albedo,
+                                 float* d_emission, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,8 @@ class Oracle:
             "orc_gen_material": (_U32, [_P, _P, _I64, _I64, _I64]),
             "orc_gen_dense": (None, [_P, _P, ctypes.c_int]),
             "orc_compare_dense": (_U64, [_P, ctypes.c_int, _P, _U32, ctypes.c_int]),
+            "orc_pathtrace": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int]),
+            "orc_pathtrace_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -147,6 +149,31 @@ class Oracle:
 
     def gen_height(self, params, x, z):
         return self.L.orc_gen_height(ctypes.byref(params), int(x), int(z))
+
+    def pathtrace(self, nodes, root, params, threads=8):
+        """FP32 image (height, width, 3); `params` is a ctypes struct laid out as orc_pt_params."""
+        nodes = self._nodes(nodes)
+        sub = self.find_subdags(nodes, root)
+        img = np.zeros((params.height, params.width, 3), np.float32)
+        if self.L.orc_pathtrace(_ptr(nodes), _ptr(sub), ctypes.byref(params), _ptr(img), int(threads)) != 0:
+            raise ValueError("bad path-tracer parameters")
+        return img
+
+    def pathtrace_f64(self, nodes, root, params, d_image=None, threads=8):
+        """FP64 image and, if d_image is given, the gradient of sum(d_image * image) with respect to
+        the albedo and emission tables (n_materials x 3 each)."""
+        nodes = self._nodes(nodes)
+        sub = self.find_subdags(nodes, root)
+        img = np.zeros((params.height, params.width, 3), np.float64)
+        m = params.n_materials
+        da = np.zeros((m, 3), np.float64)
+        de = np.zeros((m, 3), np.float64)
+        w = None if d_image is None else np.ascontiguousarray(d_image, dtype=np.float64)
+        rc = self.L.orc_pathtrace_f64(_ptr(nodes), _ptr(sub), ctypes.byref(params),
+                                      None if w is None else _ptr(w), _ptr(img), _ptr(da), _ptr(de), int(threads))
+        if rc != 0:
+            raise ValueError("bad path-tracer parameters")
+        return img, da, de
 
     def compare_dense(self, grid, log2_size, nodes, root, threads=8):
         nodes = self._nodes(nodes)

@@ -275,7 +275,7 @@ static int esvo(walk_ctx* w, uint32_t node, const int32_t pos[3], int height, co
 }
 
 /* intersectVolume, src/library/raytracing.cpp:407-478. */
-static void intersect_one(const uint32_t* nodes, const orc_subdag* subdags, const float* r,
+void orc_intersect_one(const uint32_t* nodes, const orc_subdag* subdags, const float* r,
                           int surf, float max_footprint, uint32_t max_iters, orc_hit* hit)
 {
     memset(hit, 0, sizeof(*hit));
@@ -331,7 +331,7 @@ void orc_intersect(const uint32_t* nodes, const orc_subdag* subdags, const float
                    int surf, float max_footprint, uint32_t max_iters, orc_hit* out)
 {
     for (uint64_t i = 0; i < n; i++)
-        intersect_one(nodes, subdags, rays + 6 * i, surf, max_footprint, max_iters, &out[i]);
+        orc_intersect_one(nodes, subdags, rays + 6 * i, surf, max_footprint, max_iters, &out[i]);
 }
 
 typedef struct {
@@ -355,7 +355,7 @@ static void* mt_worker(void* arg)
         uint64_t c1 = c0 + j->chunk < j->n ? c0 + j->chunk : j->n;
         for (uint64_t i = c0; i < c1; i++) {
             orc_hit* h = j->out ? &j->out[i] : &scratch;
-            intersect_one(j->nodes, j->subdags, j->rays + 6 * i, j->surf, j->max_footprint, j->max_iters, h);
+            orc_intersect_one(j->nodes, j->subdags, j->rays + 6 * i, j->surf, j->max_footprint, j->max_iters, h);
             hits += h->info & 1u;
         }
     }

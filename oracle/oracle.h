@@ -51,6 +51,9 @@ typedef struct {
 
 /* ---- traversal (cpu_ref.c) ---- */
 int orc_find_subdags(const uint32_t* nodes, uint64_t node_count, uint32_t root, orc_subdag out[8]);
+/* one ray: r = {ox, oy, oz, dx, dy, dz} */
+void orc_intersect_one(const uint32_t* nodes, const orc_subdag* subdags, const float* r, int surf,
+                       float max_footprint, uint32_t max_iters, orc_hit* hit);
 void orc_intersect(const uint32_t* nodes, const orc_subdag* subdags, const float* rays, uint64_t n,
                    int surf, float max_footprint, uint32_t max_iters, orc_hit* out);
 void orc_intersect_mt(const uint32_t* nodes, const orc_subdag* subdags, const float* rays, uint64_t n,
@@ -90,6 +93,29 @@ uint32_t orc_gen_material(const orc_gen_params* p, const double* spheres, int64_
 void orc_gen_dense(const orc_gen_params* p, uint8_t* grid, int threads);
 /* number of voxels whose material differs between the dense grid and the DAG */
 uint64_t orc_compare_dense(const uint8_t* grid, int log2_size, const uint32_t* nodes, uint32_t root, int threads);
+
+/* ---- path tracer restatement (cpu_pt.c) ---- */
+/* same field order as the product's cqx_pt_params */
+typedef struct {
+    uint32_t width, height, spp, bounces;
+    double eye[3], target[3], fov_y_degrees;
+    uint32_t n_materials;
+    const float* albedo;
+    const float* emission;
+    float sky[3];
+    float bounce_offset;
+    float max_footprint;
+    uint64_t seed;
+} orc_pt_params;
+
+/* FP32 forward image (height x width x 3), arithmetic identical to the GPU renderer */
+int orc_pathtrace(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt_params* p, float* image,
+                  int threads);
+/* FP64 reference: image and the gradient of sum_pixels w . image with respect to the material
+ * tables (w = d_image, height x width x 3), over the same paths (geometry in FP32, exactly as the
+ * GPU traces it; throughput, radiance and gradients in FP64). Any output may be NULL. */
+int orc_pathtrace_f64(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt_params* p,
+                      const double* d_image, double* image, double* d_albedo, double* d_emission, int threads);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,182 @@
+"""Path tracer and path-replay adjoint.
+
+CPU (oracle pinning): the FP64 oracle gradient against central finite differences of the FP64
+image (the image is a polynomial in the tables, so FD with h = 1e-3 is accurate); closed forms
+(an empty volume shows the sky; emission enters linearly); FP32 oracle image against FP64.
+GPU: both renderers (hipGraph wavefront, megakernel) bit-identical to the FP32 oracle image
+(stated tolerance: RMSE <= 1e-6 of the mean radiance; observed 0); graph replays interleaved with
+eager work all equal; the adjoint within 1e-4 relative of the FP64 oracle gradient and within
+1e-3 relative of central finite differences (h = 1e-3) of the GPU loss.
+"""
+import numpy as np
+import pytest
+
+from cubiquity_amd import workloads as W
+
+
+@pytest.fixture(scope="module")
+def scene(cq):
+    hv = cq.HostVolume.generate(cq.gen_default_params(7, 1, n_materials=9))
+    alb, em = cq.material_table(9, 11)
+    setup = cq.PathTraceSetup(W.RAYCAST_128.camera, 96, 64, 4, 4, alb, em)
+    return hv, setup
+
+
+@pytest.fixture(scope="module")
+def scene16(cq):
+    """A reduced differentiable-rendering case: 64^3, 16 materials, 3 bounces."""
+    hv = cq.HostVolume.generate(cq.gen_default_params(6, 5, n_materials=16))
+    cam = W._camera(6, 48, 48)
+    alb, em = cq.material_table(16, 11)
+    tgt_alb, _ = cq.material_table(16, 12)
+    setup = cq.PathTraceSetup(cam, 48, 48, 4, 3, alb, em)
+    return hv, setup, tgt_alb
+
+
+def _fd_step(x, h):
+    """float32 perturbations and the actual step between them."""
+    xp = np.float32(x + h)
+    xm = np.float32(x - h)
+    return xp, xm, float(xp) - float(xm)
+
+
+# ---- CPU: pin the oracle ------------------------------------------------------------------------
+
+def test_oracle_fp32_vs_fp64_image(oracle, scene):
+    hv, s = scene
+    a = oracle.pathtrace(hv.nodes, hv.root, s.params)
+    b, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, s.params)
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+    assert a.mean() > 0.05  # the scene is lit
+
+
+def test_oracle_empty_volume_is_sky(cq, oracle):
+    hv = cq.HostVolume.from_dense(np.zeros((8, 8, 8), np.uint8))
+    alb, em = cq.material_table(3, 1)
+    s = cq.PathTraceSetup(W.RAYCAST_128.camera, 16, 12, 2, 3, alb, em, sky=(0.8, 0.7, 0.25))
+    img = oracle.pathtrace(hv.nodes, hv.root, s.params)
+    assert (img == np.asarray([0.8, 0.7, 0.25], np.float32)).all()
+
+
+def test_oracle_emission_is_linear(oracle, scene):
+    hv, s = scene
+    sky0 = s.with_tables(s.albedo, s.emission)
+    sky0.params.sky[0] = sky0.params.sky[1] = sky0.params.sky[2] = 0.0
+    a, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, sky0.params)
+    e2 = sky0.with_tables(s.albedo, 2 * s.emission)
+    e2.params.sky[0] = e2.params.sky[1] = e2.params.sky[2] = 0.0
+    b, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, e2.params)
+    assert np.allclose(b, 2 * a, rtol=1e-12, atol=1e-12)
+    assert a.max() > 0
+
+
+def test_oracle_gradient_vs_finite_differences(oracle, scene16):
+    hv, s, tgt_alb = scene16
+    target, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, s.with_tables(tgt_alb, s.emission).params)
+    img, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, s.params)
+    _, da, de = oracle.pathtrace_f64(hv.nodes, hv.root, s.params, 2 * (img - target))
+
+    def loss(a, e):
+        im, _, _ = oracle.pathtrace_f64(hv.nodes, hv.root, s.with_tables(a, e).params)
+        return ((im - target) ** 2).sum()
+
+    rng = np.random.Generator(np.random.PCG64(4))
+    for table, grad in (("albedo", da), ("emission", de)):
+        for m, c in zip(rng.integers(0, 16, 6), rng.integers(0, 3, 6)):
+            a, e = s.albedo.copy(), s.emission.copy()
+            t = a if table == "albedo" else e
+            xp, xm, step = _fd_step(t[m, c], 1e-3)
+            t[m, c] = xp
+            lp = loss(a, e)
+            t[m, c] = xm
+            lm = loss(a, e)
+            fd = (lp - lm) / step
+            scale = max(abs(fd), 1e-3 * np.abs(grad).max())
+            assert abs(grad[m, c] - fd) <= 1e-4 * scale, (table, m, c, grad[m, c], fd)
+
+
+# ---- GPU ------------------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def dscene(gpu_ctx, scene):
+    hv, s = scene
+    return gpu_ctx.upload(hv)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_image_matches_oracle(cq, oracle, gpu_ctx, scene, dscene, mode):
+    hv, s = scene
+    img = cq.pathtrace(gpu_ctx, dscene, s, mode).cpu().numpy()
+    ref = oracle.pathtrace(hv.nodes, hv.root, s.params)
+    rmse = float(np.sqrt(((img.astype(np.float64) - ref) ** 2).mean()))
+    assert rmse <= 1e-6 * float(np.abs(ref).mean()), rmse
+    assert np.array_equal(img, ref), f"{(img != ref).sum()} values differ (rmse {rmse})"
+
+
+@pytest.mark.gpu
+def test_gpu_graph_replays_match_eager(cq, gpu_ctx, scene, dscene):
+    """The captured wavefront graph, replayed with eager kernels and syncs in between."""
+    import torch
+
+    hv, s = scene
+    eager = cq.pathtrace(gpu_ctx, dscene, s, 1).clone()
+    for k in range(4):
+        img = cq.pathtrace(gpu_ctx, dscene, s, 0).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(img, eager), f"replay {k}"
+        cq.pathtrace(gpu_ctx, dscene, s, 1)  # eager work between replays
+        torch.cuda.synchronize()
+    # new tables reuse the same graph (they live in a parameter block, not in the graph)
+    s2 = s.with_tables(s.albedo * 0.5, s.emission)
+    a = cq.pathtrace(gpu_ctx, dscene, s2, 0).clone()
+    b = cq.pathtrace(gpu_ctx, dscene, s2, 1).clone()
+    assert torch.equal(a, b) and not torch.equal(a, eager)
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_vs_fp64_and_fd(cq, oracle, gpu_ctx, scene16):
+    import torch
+
+    hv, s, tgt_alb = scene16
+    dv = gpu_ctx.upload(hv)
+    st = s.with_tables(tgt_alb, s.emission)
+    target = cq.pathtrace(gpu_ctx, dv, st, 1).double()
+    img = cq.pathtrace(gpu_ctx, dv, s, 1).double()
+    d_image = (2 * (img - target)).float()
+    da, de = cq.pathtrace_adjoint(gpu_ctx, dv, s, d_image)
+    da, de = da.cpu().numpy().astype(np.float64), de.cpu().numpy().astype(np.float64)
+    # FP64 reference gradient over the same paths, same dLoss/dimage
+    _, ra, re = oracle.pathtrace_f64(hv.nodes, hv.root, s.params, d_image.cpu().numpy().astype(np.float64))
+    g = np.concatenate([da.ravel(), de.ravel()])
+    r = np.concatenate([ra.ravel(), re.ravel()])
+    scale = np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
+    assert (np.abs(g - r) <= 1e-4 * scale).all(), np.abs(g - r).max()
+
+    def loss(a, e):
+        im = cq.pathtrace(gpu_ctx, dv, s.with_tables(a, e), 1).double()
+        return float(((im - target) ** 2).sum())
+
+    # central differences of the GPU loss on all 96 parameters; relative error per parameter
+    # against max(|fd|, 1e-2 max|fd|) and of the whole gradient vector (L2)
+    fds, gs = [], []
+    for table, grad in (("albedo", da), ("emission", de)):
+        for m in range(16):
+            for c in range(3):
+                a, e = s.albedo.copy(), s.emission.copy()
+                t = a if table == "albedo" else e
+                xp, xm, step = _fd_step(t[m, c], 1e-3)
+                t[m, c] = xp
+                lp = loss(a, e)
+                t[m, c] = xm
+                lm = loss(a, e)
+                fds.append((lp - lm) / step)
+                gs.append(grad[m, c])
+    fds, gs = np.asarray(fds), np.asarray(gs)
+    sc = np.maximum(np.abs(fds), 1e-2 * np.abs(fds).max())
+    per = np.abs(gs - fds) / sc
+    l2 = np.linalg.norm(gs - fds) / np.linalg.norm(fds)
+    print(f"adjoint vs FD: max per-parameter {per.max():.2e}, L2 {l2:.2e}", flush=True)
+    assert per.max() <= 1e-3 and l2 <= 1e-3, (per.max(), l2)
+    dv.free()
+    torch.cuda.synchronize()
