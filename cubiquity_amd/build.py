@@ -78,7 +78,19 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-pthread"], verbose)
     os.replace(tmp, LIB)
+    build_cpp_checks(verbose)
     return LIB
+
+
+def build_cpp_checks(verbose=False):
+    """The C++ drop-in example run by the GPU tests (links libcqx.so through its rpath)."""
+    src = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "dropin_check")
+    _run([CXX, "-std=c++17", "-O2", f"-I{INCLUDE}", src, "-o", out, f"-L{HERE}", "-lcqx",
+          "-Wl,-rpath,$ORIGIN/../../cubiquity_amd"], verbose)
+    return out
 
 
 if __name__ == "__main__":
