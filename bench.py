@@ -117,6 +117,42 @@ def bench_pathtrace(cq, W, ctx, reps):
     return out
 
 
+def bench_batched(cq, W, ctx, reps):
+    """64 volumes (seeds 100..163, sides 2^(6 + i mod 6)) with 64 ray batches of log-uniform size in
+    [1e3, 1e6] (even: primary camera rays, odd: incoherent rays): one persistent launch against 64
+    separate launches."""
+    import torch
+
+    t0 = time.time()
+    vols = W.batched_volumes()
+    sizes = W.batched_sizes()
+    items, dvols = [], []
+    for i, (w, n) in enumerate(zip(vols, sizes)):
+        hv = cq.HostVolume.generate(w.params())
+        dv = ctx.upload(hv)
+        if i % 2 == 0:
+            side = int(np.ceil(np.sqrt(n)))
+            rays = cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, side, side)[:n]
+        else:
+            rays = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, n)
+        r = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+        items.append((dv, r, torch.empty((n, 32), dtype=torch.uint8, device="cuda"), True, -1.0))
+        dvols.append(dv)
+    total = int(sum(sizes))
+    log(f"batched: 64 volumes built and uploaded in {time.time() - t0:.1f}s, {total} rays")
+    batched = timed(lambda: cq.intersect_batched(ctx, items), reps)
+
+    def separate():
+        for dv, r, h, s, fp in items:
+            cq.intersectVolume(ctx, dv, r, s, fp, out=h)
+
+    sep = timed(separate, reps)
+    for dv in dvols:
+        dv.free()
+    return {"volumes": 64, "batches": 64, "rays": total, "batched_ms": round(batched, 3),
+            "separate_launches_ms": round(sep, 3), "mrays_per_s": round(total / (batched * 1e-3) / 1e6, 1)}
+
+
 def bench_adjoint(cq, W, ctx, reps):
     """Differentiable rendering (512^3, 16 materials, 512x512, 8 spp, 3 bounces): forward time,
     adjoint (path replay) time and their ratio."""
@@ -160,6 +196,10 @@ def main():
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pathtrace", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB before timing (a known-size "
+                         "kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard by "
@@ -185,6 +225,13 @@ def main():
     stream = torch.cuda.current_stream()
     cfg = cq.LaunchCfg(variant=args.variant)
 
+    if args.calibration_copy_mib:
+        nbytes = args.calibration_copy_mib << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=cfg, out=d_hits)
     torch.cuda.synchronize()
@@ -256,6 +303,9 @@ def main():
         out["adjoint_forward_ratio"] = adj["ratio"]
         out["diffrender_512"] = adj
         log(f"adjoint: {adj}")
+    if not args.no_batched:
+        out["batched"] = bench_batched(cq, W, ctx, max(2, args.steps // 4))
+        log(f"batched: {out['batched']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hv, rays, hits, cq)
         out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

@@ -240,6 +240,28 @@ class Context:
             pass
 
 
+def intersect_batched(ctx, items, cfg=None, stream=None):
+    """One persistent launch over many batches. `items`: (DeviceVolume, rays GPU tensor (n, 6),
+    hits GPU uint8 tensor (n, 32), computeSurfaceProperties, maxFootprint) tuples."""
+    import torch
+
+    arr = (cabi.Batch * max(1, len(items)))()
+    for i, (dvol, rays, hits, surf, fp) in enumerate(items):
+        n = rays.shape[0]
+        if hits.numel() < n * 32:
+            raise ValueError(f"hit buffer of batch {i} too small")
+        arr[i].volume = dvol._h.value if isinstance(dvol._h, ctypes.c_void_p) else dvol._h
+        arr[i].rays = rays.data_ptr()
+        arr[i].hits = hits.data_ptr()
+        arr[i].n = n
+        arr[i].flags = CQX_SURFACE_PROPERTIES if surf else 0
+        arr[i].max_footprint = float(fp)
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    cfgp = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().cqx_intersect_batched(ctx._h, arr, len(items), cfgp, ctypes.c_void_p(stream)))
+
+
 def material_table(n_materials, seed, n_emissive=1, emission=5.0):
     """Albedo U[0.1, 0.9]^3 per material (numpy PCG64(seed)); the last `n_emissive` materials
     emit `emission` on every channel. Rows are material ids 1..n_materials."""

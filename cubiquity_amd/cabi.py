@@ -55,6 +55,14 @@ class PtParams(ctypes.Structure):
 
 assert ctypes.sizeof(PtParams) == 128
 
+
+class Batch(ctypes.Structure):
+    _fields_ = [("volume", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p),
+                ("n", ctypes.c_uint64), ("flags", ctypes.c_uint32), ("max_footprint", ctypes.c_float)]
+
+
+assert ctypes.sizeof(Batch) == 40
+
 _LIB = None
 
 _P = ctypes.c_void_p
@@ -90,6 +98,7 @@ SIGNATURES = {
     "cqx_volume_info": (_ST, [_P, _P, _P, _P, _P]),
     "cqx_intersect": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_host": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P]),
+    "cqx_intersect_batched": (_ST, [_P, _P, _U32, _P, _P]),
     "cqx_pathtrace": (_ST, [_P, _P, _P, _U32, _P, _P, _P]),
     "cqx_pathtrace_adjoint": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }

@@ -35,6 +35,8 @@ struct cqx_ctx {
     size_t scratch_bytes = 0;
     hipStream_t capture_stream = nullptr; // private stream for graph capture
     cqx::PtPlan* pt_plan = nullptr;
+    void* batch_desc = nullptr; // descriptor table of the batched launch
+    size_t batch_bytes = 0;
 };
 
 struct cqx_dvolume {

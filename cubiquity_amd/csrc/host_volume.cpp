@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <thread>
@@ -25,9 +26,14 @@ cqx_status fail(cqx_status code, const char* fmt, ...)
 
 void clear_error() { g_last_error.clear(); }
 
+// All cores, or OMP_NUM_THREADS when the environment sets a CPU share.
 int default_threads()
 {
     unsigned n = std::thread::hardware_concurrency();
+    if (const char* env = std::getenv("OMP_NUM_THREADS")) {
+        const int k = std::atoi(env);
+        if (k > 0 && (n == 0 || unsigned(k) < n)) n = unsigned(k);
+    }
     return n ? int(n) : 1;
 }
 

@@ -368,16 +368,26 @@ __global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restric
         partials[uint64_t(blockIdx.x) * M * 6 + k] = g[6 + k]; // material ids 1..M
 }
 
-__global__ void k_pt_reduce(const float* __restrict__ partials, uint32_t blocks, uint32_t M, float* __restrict__ d_albedo,
-                            float* __restrict__ d_emission)
+// One workgroup per output value ((material - 1) * 6 + component): strided FP64 sums over the
+// block partials, then a fixed-order tree in LDS (deterministic for a given grid).
+__global__ void __launch_bounds__(256) k_pt_reduce(const float* __restrict__ partials, uint32_t blocks, uint32_t M,
+                                                   float* __restrict__ d_albedo, float* __restrict__ d_emission)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; // (material - 1) * 6 + component
-    if (k >= M * 6) return;
+    __shared__ double acc[256];
+    const uint32_t k = blockIdx.x;
     double sum = 0.0;
-    for (uint32_t b = 0; b < blocks; b++) sum += double(partials[uint64_t(b) * M * 6 + k]);
-    const uint32_t m = k / 6, c = k % 6;
-    if (c < 3) d_albedo[m * 3 + c] = float(sum);
-    else d_emission[m * 3 + (c - 3)] = float(sum);
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) sum += double(partials[uint64_t(b) * M * 6 + k]);
+    acc[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t w = blockDim.x / 2; w > 0; w /= 2) {
+        if (threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t m = k / 6, c = k % 6;
+        if (c < 3) d_albedo[m * 3 + c] = float(acc[0]);
+        else d_emission[m * 3 + (c - 3)] = float(acc[0]);
+    }
 }
 
 } // namespace
@@ -660,7 +670,7 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), lds, s, p->dparams, trace_params(vol, pp, cfg), p->P,
                        d_image, pp->n_materials, p->partials);
-    hipLaunchKernelGGL(k_pt_reduce, dim3((pp->n_materials * 6 + 255) / 256), dim3(256), 0, s, p->partials, blocks,
+    hipLaunchKernelGGL(k_pt_reduce, dim3(pp->n_materials * 6), dim3(256), 0, s, p->partials, blocks,
                        pp->n_materials, d_albedo, d_emission);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <new>
+#include <vector>
 
 using namespace cqxd;
 using cqx::fail;
@@ -52,9 +53,57 @@ __global__ void k_reset_counters(unsigned long long* counters)
     if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, const cqx_ray* __restrict__ rays,
-                                                              uint64_t n, const uint32_t* __restrict__ n_dev,
-                                                              cqx_hit* __restrict__ hits,
+// Where a persistent wave's ray ids come from. SingleSource: one volume, one ray array; its
+// parameters stay wave-uniform. BatchSource: many (volume, ray array) batches laid end to end in
+// one id space; a lane finds its batch by binary search over the batches' first ids.
+struct SingleSource {
+    TraceParams P;
+    const cqx_ray* __restrict__ rays;
+    cqx_hit* __restrict__ hits;
+    __device__ __forceinline__ void fetch(uint64_t id, TraceParams& T, const float*& ray, cqx_hit*& out) const
+    {
+        T = P;
+        ray = reinterpret_cast<const float*>(rays + id);
+        out = hits + id;
+    }
+};
+
+struct BatchDesc {
+    const uint32_t* nodes;
+    const cqx_subdag* sub;
+    const cqx_ray* rays;
+    cqx_hit* hits;
+    uint64_t first; // first id of this batch in the launch's id space
+    uint32_t flags;
+    float max_footprint;
+};
+
+struct BatchSource {
+    const BatchDesc* __restrict__ desc;
+    uint32_t n_batches;
+    uint32_t max_iters;
+    __device__ __forceinline__ void fetch(uint64_t id, TraceParams& T, const float*& ray, cqx_hit*& out) const
+    {
+        uint32_t lo = 0, hi = n_batches; // invariant: desc[lo].first <= id < desc[hi].first
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (desc[mid].first <= id) lo = mid;
+            else hi = mid;
+        }
+        const BatchDesc& b = desc[lo];
+        T.nodes = b.nodes;
+        T.sub = b.sub;
+        T.surf = (b.flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
+        T.max_footprint = b.max_footprint;
+        T.max_iters = max_iters;
+        const uint64_t k = id - b.first;
+        ray = reinterpret_cast<const float*>(b.rays + k);
+        out = b.hits + k;
+    }
+};
+
+template <typename Source>
+__global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
                                                               uint32_t regroup, uint32_t refill)
 {
@@ -64,6 +113,8 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, con
         const uint64_t nd = *n_dev;
         n = nd < n ? nd : n;
     }
+    TraceParams P;
+    cqx_hit* dst = nullptr;
     uint32_t* stack = lds_stack + threadIdx.x;
     const uint32_t stride = blockDim.x;
     const uint32_t lane = threadIdx.x & 63u;
@@ -73,7 +124,6 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, con
 
     RayState S;
     S.phase = kIdle;
-    uint64_t rid = 0;
     cqx_hit out;
     bool exhausted = false;
 
@@ -96,8 +146,8 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, con
                 const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 const uint64_t k = base + rank;
                 if (k < q_len) {
-                    rid = q_begin + k;
-                    const float* r = reinterpret_cast<const float*>(rays + rid);
+                    const float* r;
+                    src.fetch(q_begin + k, P, r, dst);
                     ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
                 }
             }
@@ -106,10 +156,25 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(TraceParams P, con
             if (S.phase != kIdle) {
                 const StepResult res = (S.phase == kOctant) ? octant_enter(P, S, stack, stride, out)
                                                             : walk_step(P, S, stack, stride, out);
-                if (res == kDone) hits[rid] = out;
+                if (res == kDone) *dst = out;
             }
         }
     }
+}
+
+// Launch shape of the persistent kernel: CUs x waves_per_cu waves, at most one lane per ray.
+cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint32_t block, uint64_t n,
+                            uint32_t& blocks, uint32_t& regroup, uint32_t& refill)
+{
+    const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
+    regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
+    refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
+    if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
+        return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
+    const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
+    const uint64_t need_blocks = (n + block - 1) / block;
+    blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
+    return CQX_OK;
 }
 
 cqx_status check_ctx(cqx_ctx* ctx)
@@ -147,17 +212,13 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
         hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
     } else if (variant == 2) {
-        const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
-        const uint32_t regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
-        const uint32_t refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
-        if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
-            return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
-        const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
-        const uint64_t need_blocks = (n + block - 1) / block;
-        const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
+        uint32_t blocks, regroup, refill;
+        cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill);
+        if (st != CQX_OK) return st;
+        SingleSource src{ P, rays, hits };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter));
-        hipLaunchKernelGGL(k_intersect_persistent, dim3(blocks), dim3(block), lds, stream, P, rays, n, n_dev, hits,
+        hipLaunchKernelGGL(k_intersect_persistent<SingleSource>, dim3(blocks), dim3(block), lds, stream, src, n, n_dev,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
     } else {
         return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
@@ -206,6 +267,7 @@ cqx_status cqx_destroy(cqx_ctx* ctx)
     cqx::destroy_pt_plan(ctx->pt_plan);
     release(ctx->d_counter);
     release(ctx->scratch);
+    release(ctx->batch_desc);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
     return CQX_OK;
@@ -293,6 +355,61 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
     if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
     if (n > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "%llu rays exceed the 2^38 limit", (unsigned long long)n);
     return cqx::launch_intersect(ctx, vol, rays, n, nullptr, flags, max_footprint, hits, cfg, hipStream_t(stream));
+}
+
+cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_t n_batches, const cqx_launch_cfg* cfg,
+                                 void* stream_ptr)
+{
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    if (n_batches == 0) return CQX_OK;
+    if (!batches) return fail(CQX_ERR_INVALID_ARG, "null batch array");
+    std::vector<BatchDesc> desc(n_batches);
+    uint64_t total = 0;
+    int max_height = 0;
+    for (uint32_t i = 0; i < n_batches; i++) {
+        const cqx_batch& b = batches[i];
+        if (!b.volume) return fail(CQX_ERR_INVALID_ARG, "batch %u has no volume", i);
+        if (b.flags & ~CQX_SURFACE_PROPERTIES) return fail(CQX_ERR_INVALID_ARG, "batch %u: unknown flags", i);
+        if (b.n && (!b.rays || !b.hits)) return fail(CQX_ERR_INVALID_ARG, "batch %u: null ray or hit buffer", i);
+        desc[i].nodes = b.volume->nodes;
+        desc[i].sub = b.volume->sub;
+        desc[i].rays = b.rays;
+        desc[i].hits = b.hits;
+        desc[i].first = total;
+        desc[i].flags = b.flags;
+        desc[i].max_footprint = b.max_footprint;
+        total += b.n;
+        if (b.n) max_height = std::max(max_height, b.volume->max_height);
+    }
+    if (total == 0) return CQX_OK;
+    if (total > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "more than 2^38 rays in one launch");
+    const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
+    if (block % 64 != 0 || block == 0 || block > 256) return fail(CQX_ERR_INVALID_ARG, "bad block size %u", block);
+    const size_t lds = size_t(max_height + 1) * block * sizeof(uint32_t);
+    if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
+    hipStream_t s = hipStream_t(stream_ptr);
+    // The descriptor table is per context: wait for an earlier batched launch on this stream.
+    HIP_TRY(hipStreamSynchronize(s));
+    const size_t bytes = desc.size() * sizeof(BatchDesc);
+    if (bytes > ctx->batch_bytes) {
+        release(ctx->batch_desc);
+        ctx->batch_desc = nullptr;
+        ctx->batch_bytes = 0;
+        if (hipMalloc(&ctx->batch_desc, bytes) != hipSuccess) return fail(CQX_ERR_OOM, "descriptor allocation failed");
+        ctx->batch_bytes = bytes;
+    }
+    HIP_TRY(hipMemcpy(ctx->batch_desc, desc.data(), bytes, hipMemcpyHostToDevice));
+    uint32_t blocks, regroup, refill;
+    st = persistent_shape(ctx, cfg, block, total, blocks, regroup, refill);
+    if (st != CQX_OK) return st;
+    BatchSource src{ static_cast<const BatchDesc*>(ctx->batch_desc), n_batches,
+                     (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters };
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(ctx->d_counter));
+    hipLaunchKernelGGL(k_intersect_persistent<BatchSource>, dim3(blocks), dim3(block), lds, s, src, total,
+                       (const uint32_t*)nullptr, reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
+    HIP_TRY(hipGetLastError());
+    return CQX_OK;
 }
 
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,

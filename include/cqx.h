@@ -209,6 +209,24 @@ typedef struct {
 cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                          uint32_t flags, float max_footprint, cqx_hit* hits,
                          const cqx_launch_cfg* cfg, void* stream);
+/* Many (volume, ray batch) pairs in ONE persistent launch: the batches are laid end to end in
+ * one ray-id space that the persistent waves drain together, so batches of very different sizes
+ * and volumes of different depths share the GPU without a launch per batch. Per-ray semantics
+ * are intersectVolume's (raytracing.cpp:397-478). rays/hits are device pointers. One batched
+ * launch in flight per context. (No reference counterpart: the reference raycasts one ray per
+ * call.) */
+typedef struct {
+    const cqx_dvolume* volume;
+    const cqx_ray* rays;
+    cqx_hit* hits;
+    uint64_t n;
+    uint32_t flags;
+    float max_footprint;
+} cqx_batch;
+
+cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_t n_batches,
+                                 const cqx_launch_cfg* cfg, void* stream);
+
 /* Same with host buffers: copies in, runs, copies out, synchronizes. */
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                               uint32_t flags, float max_footprint, cqx_hit* hits);
