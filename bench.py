@@ -197,6 +197,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch defaults)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing (a known-size "
                          "kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE)")
@@ -232,6 +233,15 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    tuned = None
+    if args.variant == 0 and not args.no_tune:
+        # the autotuner picks the launch on these rays; later auto launches (cfg variant 0) use it
+        t0 = time.time()
+        best, best_ms = cq.autotune(ctx, dvol, d_rays, True, -1.0)
+        tuned = {f: getattr(best, f) for f, _ in cq.LaunchCfg._fields_}
+        tuned["tuning_s"] = round(time.time() - t0, 2)
+        tuned["tuned_ms"] = round(best_ms, 4)
+        log(f"autotuned launch: {tuned}")
     for _ in range(args.warmup):
         cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=cfg, out=d_hits)
     torch.cuda.synchronize()
@@ -277,6 +287,7 @@ def main():
             "rays": args.rays,
             "nodes": hv.node_count,
             "variant": args.variant,
+            "launch": tuned if tuned else "explicit variant" if args.variant else "built-in defaults",
         },
         "roofline": {
             "bound": "hbm",

@@ -240,6 +240,25 @@ class Context:
             pass
 
 
+def autotune(ctx, dvol, rays, computeSurfaceProperties=True, maxFootprint=MAX_FOOTPRINT_DISABLED, stream=None):
+    """Tune the traversal launch on these GPU rays (cqx_autotune); returns (LaunchCfg, ms). The
+    choice is cached in the context and used by later auto-configured launches."""
+    import torch
+
+    rays = rays.contiguous().view(-1, 6)
+    n = rays.shape[0]
+    hits = torch.empty((n, 32), dtype=torch.uint8, device=rays.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(rays.device).cuda_stream
+    best = LaunchCfg()
+    ms = ctypes.c_float()
+    check(lib().cqx_autotune(ctx._h, dvol._h, ctypes.c_void_p(rays.data_ptr()), ctypes.c_uint64(n),
+                             ctypes.c_uint32(CQX_SURFACE_PROPERTIES if computeSurfaceProperties else 0),
+                             ctypes.c_float(maxFootprint), ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream),
+                             ctypes.byref(best), ctypes.byref(ms)))
+    return best, float(ms.value)
+
+
 def intersect_batched(ctx, items, cfg=None, stream=None):
     """One persistent launch over many batches. `items`: (DeviceVolume, rays GPU tensor (n, 6),
     hits GPU uint8 tensor (n, 32), computeSurfaceProperties, maxFootprint) tuples."""

@@ -99,6 +99,8 @@ SIGNATURES = {
     "cqx_intersect": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_host": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P]),
     "cqx_intersect_batched": (_ST, [_P, _P, _U32, _P, _P]),
+    "cqx_autotune": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P, _P]),
+    "cqx_autotune_clear": (_ST, [_P]),
     "cqx_pathtrace": (_ST, [_P, _P, _P, _U32, _P, _P, _P]),
     "cqx_pathtrace_adjoint": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }

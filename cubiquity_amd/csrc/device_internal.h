@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <vector>
 
 #define HIP_TRY(expr)                                                                                            \
     do {                                                                                                         \
@@ -37,6 +38,13 @@ struct cqx_ctx {
     cqx::PtPlan* pt_plan = nullptr;
     void* batch_desc = nullptr; // descriptor table of the batched launch
     size_t batch_bytes = 0;
+    struct Tuned {
+        int height; // SubDAG height of the volume tuned on
+        int cls;    // batch-size class: floor(log2 n)
+        cqx_launch_cfg cfg;
+        float ms;
+    };
+    std::vector<Tuned> tuned; // autotuner cache
 };
 
 struct cqx_dvolume {
@@ -49,6 +57,10 @@ struct cqx_dvolume {
 };
 
 namespace cqx {
+
+// Autotuner cache lookup: the tuned configuration for a volume of this SubDAG height and a batch of
+// about n rays, or null.
+const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n);
 
 // Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,

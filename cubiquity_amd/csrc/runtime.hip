@@ -96,6 +96,7 @@ struct BatchSource {
         T.surf = (b.flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
         T.max_footprint = b.max_footprint;
         T.max_iters = max_iters;
+        T.fp_disabled = footprint_is_disabled(b.max_footprint) ? 1u : 0u;
         const uint64_t k = id - b.first;
         ray = reinterpret_cast<const float*>(b.rays + k);
         out = b.hits + k;
@@ -198,6 +199,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
     P.surf = (flags & CQX_SURFACE_PROPERTIES) ? 1u : 0u;
     P.max_footprint = max_footprint;
     P.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : kDefaultMaxIters;
+    P.fp_disabled = footprint_is_disabled(max_footprint) ? 1u : 0u;
 
     const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
     if (block % 64 != 0 || block == 0 || block > 256)
@@ -354,6 +356,15 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
     if (n == 0) return CQX_OK;
     if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
     if (n > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "%llu rays exceed the 2^38 limit", (unsigned long long)n);
+    cqx_launch_cfg eff;
+    if (!cfg || cfg->variant == 0) {
+        // "auto": the autotuner's choice for this volume depth and batch size, if there is one
+        if (const cqx_launch_cfg* t = cqx::tuned_cfg(ctx, vol->max_height, n)) {
+            eff = *t;
+            if (cfg && cfg->max_iters) eff.max_iters = cfg->max_iters;
+            cfg = &eff;
+        }
+    }
     return cqx::launch_intersect(ctx, vol, rays, n, nullptr, flags, max_footprint, hits, cfg, hipStream_t(stream));
 }
 

@@ -72,7 +72,20 @@ struct TraceParams {
     uint32_t surf;                      // computeSurfaceProperties
     float max_footprint;
     uint32_t max_iters;
+    uint32_t fp_disabled; // max_footprint is exactly -1 (MAX_FOOTPRINT_DISABLED)
 };
+
+__host__ __device__ inline bool footprint_is_disabled(float max_footprint) { return max_footprint == -1.0f; }
+
+// hasLargeFootprint = (childNodeSize / tChildExit) > maxFootprint (raytracing.cpp:276), IEEE
+// division. With the footprint test disabled (maxFootprint == -1) and a size c = 2^k > 0,
+// RN(c / t) > -1 holds exactly when t > 0, t == +0 or t < -c, which needs no division.
+__device__ __forceinline__ bool footprint_large(const TraceParams& P, int32_t csize, float tcx)
+{
+    if (P.fp_disabled && csize > 0)
+        return (tcx > 0.0f) || (tcx == 0.0f && (__float_as_uint(tcx) >> 31) == 0u) || (tcx < -float(csize));
+    return (float(csize) / tcx) > P.max_footprint;
+}
 
 enum : uint32_t { kIdle = 0, kOctant = 1, kWalk = 2 };
 enum StepResult : uint32_t { kContinue = 0, kDone = 1 };
@@ -222,8 +235,7 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
                     t0z = (float(S.cpz) - S.oz) * S.iz;
         const float tce = max3f(t0x, t0y, t0z);
         const bool internal = child >= kMaterialCount;
-        const bool large = (float(csize) / tcx) > P.max_footprint;
-        if (internal && large) {
+        if (internal && footprint_large(P, csize, tcx)) {
             if (S.height <= 1) {
                 finish(S, out, true); // deeper than a valid DAG
                 return kDone;

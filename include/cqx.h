@@ -209,6 +209,16 @@ typedef struct {
 cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                          uint32_t flags, float max_footprint, cqx_hit* hits,
                          const cqx_launch_cfg* cfg, void* stream);
+/* Launch autotuner (no reference counterpart). Times kernel variant, rays per workgroup,
+ * persistent waves per CU, regroup interval and refill threshold on these rays (device pointers;
+ * `hits` is overwritten), picks the fastest by coordinate descent and caches it in the context
+ * for this volume depth and batch-size class; later launches with cfg == NULL or variant 0 use
+ * it. Optional outputs: the chosen configuration and its median time in ms. */
+cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                        uint32_t flags, float max_footprint, cqx_hit* hits, void* stream,
+                        cqx_launch_cfg* best, float* best_ms);
+cqx_status cqx_autotune_clear(cqx_ctx* ctx);
+
 /* Many (volume, ray batch) pairs in ONE persistent launch: the batches are laid end to end in
  * one ray-id space that the persistent waves drain together, so batches of very different sizes
  * and volumes of different depths share the GPU without a launch per batch. Per-ray semantics

@@ -147,6 +147,23 @@ def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     dv.free()
 
 
+def test_gpu_autotuned_launch_is_exact(cq, oracle, gpu_ctx, vol128, rays128):
+    """The autotuner's choice is cached and used by auto launches; results stay bit-exact."""
+    import torch
+
+    ctx = cq.Context(0)
+    dv = ctx.upload(vol128)
+    rays = rays128["incoherent"]
+    r = torch.from_numpy(rays).cuda()
+    best, ms = cq.autotune(ctx, dv, r, True, -1.0)
+    assert best.variant in (1, 2) and ms > 0
+    auto = cq.intersectVolume(ctx, dv, r, True).cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
+    o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True)
+    assert records_equal(auto, o).all()
+    dv.free()
+    ctx.close()
+
+
 def test_gpu_deterministic(cq, gpu_ctx, dvol128, rays128):
     a = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
     b = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
