@@ -37,13 +37,14 @@ def cpu_threads():
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch from the committed rocprof PMC summary, if one has been recorded."""
+    """(HBM bytes per launch, L2 hit rate) from the committed rocprof PMC summary of this kernel
+    and workload (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy's readings), if recorded."""
     path = os.path.join(ROOT, "profiles", "traffic_intersect_incoherent_1024.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+    return d.get("hbm_bytes_per_launch"), d.get("l2_hit_rate")
 
 
 def cpu_baseline(hv, rays, hits_gpu, cq):
@@ -265,7 +266,7 @@ def main():
     anomalies = int(((hits["info"] >> 15) & 1).sum())
     alg_bytes = args.rays * (24 + 32) + 32 * entries
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
+    traffic, l2_hit_rate = traffic_from_profiles()
 
     value = args.rays * args.steps / wall / 1e6
     out = {
@@ -296,6 +297,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "l2_hit_rate": l2_hit_rate,
             "kernel_ms": round(avg_ms, 4),
             "bytes_per_launch": alg_bytes,
             "node_entries_per_ray": round(entries / args.rays, 3),

@@ -75,6 +75,22 @@ class HostVolume:
     def save_dag(self, path):
         check(lib().cqx_hv_save_dag(self._h, os.fsencode(path)))
 
+    def edit_spheres(self, edits):
+        """Sphere fill/carve edits (fillBrush with SphereBrush semantics). `edits`: (n, 5) array of
+        centre x, y, z, radius, material (0 carves). Returns the index of the first appended node."""
+        e = np.asarray(edits, dtype=np.float64).reshape(-1, 5)
+        arr = (cabi.SphereEdit * max(1, len(e)))()
+        for i, (x, y, z, r, m) in enumerate(e):
+            arr[i].centre = (ctypes.c_float * 3)(x, y, z)
+            arr[i].radius = r
+            arr[i].material = int(m)
+        first = ctypes.c_uint64()
+        check(lib().cqx_hv_edit_spheres(self._h, arr, ctypes.c_uint64(len(e)), ctypes.byref(first)))
+        return int(first.value)
+
+    def compact(self):
+        check(lib().cqx_hv_compact(self._h))
+
     @property
     def node_count(self):
         return int(lib().cqx_hv_node_count(self._h))
@@ -168,6 +184,15 @@ class DeviceVolume:
         self._h = handle
         self.node_count = node_count
         self.root = root
+
+    def update(self, volume, first):
+        """Append-only delta upload of `volume`'s nodes [first, node_count) (cqx_volume_update)."""
+        nodes = volume.nodes
+        check(lib().cqx_volume_update(self._ctx._h, self._h, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                      ctypes.c_uint64(nodes.shape[0]), ctypes.c_uint64(first),
+                                      ctypes.c_uint32(volume.root), None))
+        self.node_count = nodes.shape[0]
+        self.root = volume.root
 
     def info(self):
         nodes = ctypes.c_void_p()

@@ -32,6 +32,8 @@ struct HostVolume {
 std::vector<Node> material_nodes();
 
 cqx_status validate_dag(const uint32_t* nodes, uint64_t node_count, uint32_t root);
+// Breadth-first renumbering from the root; drops unreachable nodes (generator.cpp).
+void canonical_order(std::vector<Node>& nodes, uint32_t& root);
 cqx_status find_subdags(const uint32_t* nodes, uint64_t node_count, uint32_t root, cqx_subdag out[8]);
 
 int default_threads();

@@ -51,6 +51,7 @@ struct cqx_dvolume {
     uint32_t* nodes = nullptr;
     cqx_subdag* sub = nullptr;
     uint64_t node_count = 0;
+    uint64_t capacity = 0; // nodes the device buffer holds (headroom for appended edits)
     uint32_t root = 0;
     cqx_subdag hsub[8];
     int max_height = 0; // largest SubDAG start height (stack rows needed: max_height + 1)

@@ -430,10 +430,12 @@ private:
     int64_t n_[3], lo_[3];
 };
 
+} // namespace
+
 // Renumber the internal nodes breadth-first from the root (children visited in slot order, each
 // node numbered at its first visit). The result is independent of construction order (so of
 // thread scheduling), keeps the top levels together at the front, and gives the children of a
-// node neighbouring indices.
+// node neighbouring indices. Unreachable nodes are dropped.
 void canonical_order(std::vector<Node>& nodes, uint32_t& root)
 {
     if (root < kMaterialCount) {
@@ -467,7 +469,6 @@ void canonical_order(std::vector<Node>& nodes, uint32_t& root)
     root = kMaterialCount;
 }
 
-} // namespace
 } // namespace cqx
 
 using namespace cqx;

@@ -297,6 +297,7 @@ cqx_status cqx_volume_upload(cqx_ctx* ctx, const uint32_t* nodes, uint64_t node_
     cqx_dvolume* v = new (std::nothrow) cqx_dvolume;
     if (!v) return fail(CQX_ERR_OOM, "out of host memory");
     v->node_count = node_count;
+    v->capacity = node_count;
     v->root = root;
     std::memcpy(v->hsub, s, sizeof(s));
     v->max_height = 0;
@@ -316,6 +317,68 @@ cqx_status cqx_volume_upload(cqx_ctx* ctx, const uint32_t* nodes, uint64_t node_
         return fail(CQX_ERR_HIP, "upload of the node store failed");
     }
     *out = v;
+    return CQX_OK;
+}
+
+cqx_status cqx_volume_update(cqx_ctx* ctx, cqx_dvolume* vol, const uint32_t* nodes, uint64_t node_count, uint64_t first,
+                             uint32_t root, const cqx_subdag sub[8])
+{
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    if (!vol || !nodes) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    if (first > vol->node_count || first < CQX_MATERIAL_COUNT || node_count < first)
+        return fail(CQX_ERR_INVALID_ARG, "delta [%llu, %llu) does not extend the uploaded store of %llu nodes",
+                    (unsigned long long)first, (unsigned long long)node_count, (unsigned long long)vol->node_count);
+    if (node_count >= 0xffffffffull) return fail(CQX_ERR_LIMIT, "node count does not fit 32-bit indices");
+    if (root >= node_count) return fail(CQX_ERR_INVALID_DAG, "root %u out of range", root);
+    // Appended nodes may only point at material nodes or at nodes with smaller indices: with the
+    // uploaded part already validated, the store stays acyclic.
+    for (uint64_t i = first; i < node_count; i++)
+        for (int c = 0; c < 8; c++) {
+            const uint32_t ch = nodes[i * 8 + c];
+            if (ch >= CQX_MATERIAL_COUNT && ch >= i)
+                return fail(CQX_ERR_INVALID_DAG, "appended node %llu child %d = %u is not an earlier node",
+                            (unsigned long long)i, c, ch);
+        }
+    cqx_subdag s[8];
+    if (sub) {
+        std::memcpy(s, sub, sizeof(s));
+        for (int c = 0; c < 8; c++)
+            if (s[c].node >= node_count || s[c].height < 0 || s[c].height > 31)
+                return fail(CQX_ERR_INVALID_DAG, "SubDAG %d invalid", c);
+    } else {
+        st = cqx::find_subdags(nodes, node_count, root, s);
+        if (st != CQX_OK) return st;
+    }
+    int max_height = 0;
+    for (int c = 0; c < 8; c++)
+        if (s[c].node > 0) max_height = std::max(max_height, int(s[c].height));
+    HIP_TRY(hipDeviceSynchronize()); // launches in flight may read the store
+    const uint32_t* old_nodes = vol->nodes;
+    if (node_count > vol->capacity) {
+        const uint64_t cap = std::max<uint64_t>(node_count, vol->capacity + vol->capacity / 2);
+        uint32_t* p = nullptr;
+        if (hipMalloc(&p, size_t(cap) * 32) != hipSuccess) return fail(CQX_ERR_OOM, "device store growth failed");
+        if (hipMemcpy(p, vol->nodes, size_t(first) * 32, hipMemcpyDeviceToDevice) != hipSuccess) {
+            release(p);
+            return fail(CQX_ERR_HIP, "device store copy failed");
+        }
+        release(vol->nodes);
+        vol->nodes = p;
+        vol->capacity = cap;
+    }
+    if (node_count > first)
+        HIP_TRY(hipMemcpy(vol->nodes + first * 8, nodes + first * 8, size_t(node_count - first) * 32,
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(vol->sub, s, sizeof(s), hipMemcpyHostToDevice));
+    if (ctx->pt_plan && (old_nodes != vol->nodes || max_height != vol->max_height)) {
+        cqx::destroy_pt_plan(ctx->pt_plan); // its captured graph holds the old pointer / stack size
+        ctx->pt_plan = nullptr;
+    }
+    vol->node_count = node_count;
+    vol->root = root;
+    std::memcpy(vol->hsub, s, sizeof(s));
+    vol->max_height = max_height;
     return CQX_OK;
 }
 

@@ -160,6 +160,24 @@ cqx_status cqx_gen_default_params(int32_t log2_size, uint64_t seed, cqx_gen_para
  * placed at [0, N)^3. `threads` <= 0 uses all cores. */
 cqx_status cqx_hv_generate(const cqx_gen_params* p, int threads, cqx_host_volume** out);
 
+/* Sphere fill / carve (material 0) edits with the voxel outcome of the reference's
+ * fillBrush(volume, SphereBrush(centre, radius), material) applied in order
+ * (src/library/voxelization.cpp:825-915, voxelization.h:91-126), including its FP32 containment
+ * test (squared distance truncated to int64, compared with radius^2, strictly). Copy-on-write
+ * with hash-consing: no node reachable before the call changes; new nodes are appended from
+ * index *first_new on, shared when identical, and 8 equal material children collapse. */
+typedef struct {
+    float centre[3];
+    float radius;
+    uint32_t material;
+} cqx_sphere_edit;
+
+cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n,
+                               uint64_t* first_new);
+/* Drops unreachable nodes and renumbers breadth-first (what Volume::bake does for storage,
+ * storage.cpp:387-394). Invalidates device copies (upload again). */
+cqx_status cqx_hv_compact(cqx_host_volume* hv);
+
 /* Build the canonical (baked) DAG of a dense grid of material ids, x fastest then y then z,
  * whose voxel (0,0,0) sits at signed voxel coordinates `lower` (any placement, including across
  * the origin). The counterpart of filling a Volume with setVoxel and calling bake
@@ -190,6 +208,14 @@ cqx_status cqx_destroy(cqx_ctx* ctx);
 cqx_status cqx_volume_upload(cqx_ctx* ctx, const uint32_t* nodes, uint64_t node_count, uint32_t root,
                              const cqx_subdag sub[8], cqx_dvolume** out);
 cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol);
+/* Append-only delta after host edits (cqx_hv_edit_spheres): nodes [first, node_count) of the
+ * host store are copied to the device (the first `first` nodes must be unchanged since the
+ * upload), and the root and SubDAGs (NULL: recomputed) are replaced. Appended nodes may only
+ * reference material nodes or earlier nodes. Waits for the device to be idle. (The reference
+ * re-uploads nothing after edits: its GPU viewer has no onVolumeModified,
+ * gpu_pathtracing_viewer.h:14-26.) */
+cqx_status cqx_volume_update(cqx_ctx* ctx, cqx_dvolume* vol, const uint32_t* nodes, uint64_t node_count,
+                             uint64_t first, uint32_t root, const cqx_subdag sub[8]);
 /* Device pointer to the volume's node array (node_count x 8 uint32) and its node count. */
 cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, uint64_t* node_count,
                            uint32_t* root, cqx_subdag sub[8]);

@@ -54,6 +54,7 @@ class Oracle:
             "orc_trace_ref": (None, [_P, _U32, _P, _U64, _P]),
             "orc_voxel": (_U32, [_P, _U32, _I32, _I32, _I32]),
             "orc_descend": (None, [_P, _U32, _I32, _I32, _I32, _P]),
+            "orc_dense_readout": (None, [_P, _U32, _P, _P, _P]),
             "orc_time_intersect": (ctypes.c_double, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                      _U64, ctypes.c_int, _P]),
             "orc_dense_brute": (None, [_P, ctypes.c_int, _P, _U64, _P, _P, _P, _P, _P]),
@@ -112,6 +113,15 @@ class Oracle:
     def voxel(self, nodes, root, x, y, z):
         nodes = self._nodes(nodes)
         return int(self.L.orc_voxel(_ptr(nodes), root, int(x), int(y), int(z)))
+
+    def dense(self, nodes, root, lower, size):
+        """Voxels of the box [lower, lower + size) as a [z][y][x] uint8 array."""
+        nodes = self._nodes(nodes)
+        lo = np.asarray(lower, np.int32)
+        sz = np.asarray(size, np.int32)
+        out = np.zeros((size[2], size[1], size[0]), np.uint8)
+        self.L.orc_dense_readout(_ptr(nodes), root, _ptr(lo), _ptr(sz), _ptr(out))
+        return out
 
     def descend(self, nodes, root, x, y, z):
         nodes = self._nodes(nodes)
@@ -197,6 +207,8 @@ class Reference:
             "ref_volume_root": (_U32, [_P]),
             "ref_volume_set_voxel": (None, [_P, _I32, _I32, _I32, _U32]),
             "ref_volume_save": (None, [_P, ctypes.c_char_p]),
+            "ref_fill_sphere": (None, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _U32]),
+            "ref_volume_dense": (None, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P]),
             "ref_volume_bake": (None, [_P]),
             "ref_volume_count_nodes": (_U32, [_P]),
             "ref_volume_voxel": (_U32, [_P, _I32, _I32, _I32]),
@@ -241,6 +253,17 @@ class RefVolume:
 
     def save(self, path):
         self.ref.L.ref_volume_save(self.h, os.fsencode(path))
+
+    def fill_sphere(self, centre, radius, material):
+        self.ref.L.ref_fill_sphere(self.h, float(centre[0]), float(centre[1]), float(centre[2]), float(radius),
+                                   int(material))
+
+    def dense(self, lower, size):
+        """Voxels of the box [lower, lower + size) as a [z][y][x] uint8 array."""
+        out = np.zeros((size[2], size[1], size[0]), np.uint8)
+        self.ref.L.ref_volume_dense(self.h, int(lower[0]), int(lower[1]), int(lower[2]), int(size[0]), int(size[1]),
+                                    int(size[2]), _ptr(out))
+        return out
 
     def bake(self):
         self.ref.L.ref_volume_bake(self.h)

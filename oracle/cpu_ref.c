@@ -453,6 +453,16 @@ uint32_t orc_voxel(const uint32_t* nodes, uint32_t root, int32_t x, int32_t y, i
     return node;
 }
 
+/* Voxels of the box [lower, lower + size) as [z][y][x] bytes, by root descent. */
+void orc_dense_readout(const uint32_t* nodes, uint32_t root, const int32_t lower[3], const int32_t size[3], uint8_t* out)
+{
+    for (int32_t z = 0; z < size[2]; z++)
+        for (int32_t y = 0; y < size[1]; y++)
+            for (int32_t x = 0; x < size[0]; x++)
+                out[((size_t)z * size[1] + y) * size[0] + x] =
+                    (uint8_t)orc_voxel(nodes, root, lower[0] + x, lower[1] + y, lower[2] + z);
+}
+
 /* path[h] = index of the node at height h on the descent to voxel (x,y,z); heights below the
  * first material node repeat that material. */
 void orc_descend(const uint32_t* nodes, uint32_t root, int32_t x, int32_t y, int32_t z, uint32_t path[33])

@@ -61,6 +61,8 @@ void orc_intersect_mt(const uint32_t* nodes, const orc_subdag* subdags, const fl
 void orc_hits_to_results(const float* rays, const orc_hit* hits, uint64_t n, int surf, orc_result* out);
 void orc_trace_ref(const uint32_t* nodes, uint32_t root, const float* rays, uint64_t n, orc_result* out);
 uint32_t orc_voxel(const uint32_t* nodes, uint32_t root, int32_t x, int32_t y, int32_t z);
+void orc_dense_readout(const uint32_t* nodes, uint32_t root, const int32_t lower[3], const int32_t size[3],
+                       uint8_t* out);
 void orc_descend(const uint32_t* nodes, uint32_t root, int32_t x, int32_t y, int32_t z,
                  uint32_t path[33]);
 double orc_time_intersect(const uint32_t* nodes, const orc_subdag* subdags, const float* rays,

@@ -14,6 +14,7 @@
 #include "raytracing.h"
 #include "storage.h"
 #include "utility.h"
+#include "voxelization.h"
 
 #include <cstdint>
 #include <cstring>
@@ -54,6 +55,22 @@ void ref_volume_set_voxel(void* h, int32_t x, int32_t y, int32_t z, uint32_t m)
 }
 
 void ref_volume_save(void* h, const char* path) { static_cast<Volume*>(h)->save(path); }
+
+// fillBrush(volume, SphereBrush(centre, radius), material), voxelization.cpp:825-835
+void ref_fill_sphere(void* h, float cx, float cy, float cz, float radius, uint32_t m)
+{
+    fillBrush(*static_cast<Volume*>(h), SphereBrush(cx, cy, cz, radius), static_cast<MaterialId>(m));
+}
+
+// Dense read-out of a box of voxels (x fastest), via Volume::voxel.
+void ref_volume_dense(void* h, int32_t x0, int32_t y0, int32_t z0, int32_t nx, int32_t ny, int32_t nz, uint8_t* out)
+{
+    Volume* v = static_cast<Volume*>(h);
+    for (int32_t z = 0; z < nz; z++)
+        for (int32_t y = 0; y < ny; y++)
+            for (int32_t x = 0; x < nx; x++)
+                out[(size_t(z) * ny + y) * nx + x] = v->voxel(x0 + x, y0 + y, z0 + z);
+}
 
 void ref_volume_bake(void* h) { static_cast<Volume*>(h)->bake(); }
 
