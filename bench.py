@@ -199,6 +199,7 @@ def main():
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch defaults)")
+    ap.add_argument("--launch", default="", help="explicit launch, e.g. variant=2,waves_per_cu=20,regroup_interval=8")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before timing (a known-size "
                          "kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE)")
@@ -226,6 +227,11 @@ def main():
     d_hits = torch.empty((args.rays, 32), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     cfg = cq.LaunchCfg(variant=args.variant)
+    if args.launch:
+        for kv in args.launch.split(","):
+            k, v = kv.split("=")
+            setattr(cfg, k.strip(), int(v))
+        args.no_tune = True
 
     if args.calibration_copy_mib:
         nbytes = args.calibration_copy_mib << 20
@@ -288,7 +294,8 @@ def main():
             "rays": args.rays,
             "nodes": hv.node_count,
             "variant": args.variant,
-            "launch": tuned if tuned else "explicit variant" if args.variant else "built-in defaults",
+            "launch": tuned if tuned else args.launch if args.launch else
+                      "explicit variant" if args.variant else "built-in defaults",
         },
         "roofline": {
             "bound": "hbm",

@@ -5,8 +5,9 @@
 //   k_intersect_persistent  persistent waves that keep every lane busy: a lane whose ray is done
 //                           stores its record and, once enough lanes of the wave are idle,
 //                           the wave claims a contiguous block of new rays from a global counter
-//                           with one atomic and hands them out by ballot prefix (variant 2)
-// Both keep the traversal stack in LDS (one column per lane, one row per node height).
+//                           with one atomic and hands them out by ballot prefix (variant 2);
+//                           variant 3 also keeps the current node's 8 children in registers
+// All keep the traversal stack in LDS (one column per lane, one row per node height).
 #include "device_internal.h"
 
 #include <algorithm>
@@ -57,12 +58,12 @@ __global__ void k_reset_counters(unsigned long long* counters)
 // parameters stay wave-uniform. BatchSource: many (volume, ray array) batches laid end to end in
 // one id space; a lane finds its batch by binary search over the batches' first ids.
 struct SingleSource {
+    static constexpr bool kUniform = true; // P is the same for every lane (kernel argument)
     TraceParams P;
     const cqx_ray* __restrict__ rays;
     cqx_hit* __restrict__ hits;
-    __device__ __forceinline__ void fetch(uint64_t id, TraceParams& T, const float*& ray, cqx_hit*& out) const
+    __device__ __forceinline__ void fetch(uint64_t id, TraceParams&, const float*& ray, cqx_hit*& out) const
     {
-        T = P;
         ray = reinterpret_cast<const float*>(rays + id);
         out = hits + id;
     }
@@ -79,6 +80,7 @@ struct BatchDesc {
 };
 
 struct BatchSource {
+    static constexpr bool kUniform = false; // each lane's batch has its own volume and flags
     const BatchDesc* __restrict__ desc;
     uint32_t n_batches;
     uint32_t max_iters;
@@ -103,7 +105,12 @@ struct BatchSource {
     }
 };
 
-template <typename Source>
+// The parameters the traversal steps use: the kernel argument when they are the same for every
+// lane (kept in scalar registers), the lane's own copy otherwise.
+__device__ __forceinline__ const TraceParams& uniform_params(const SingleSource& s, const TraceParams&) { return s.P; }
+__device__ __forceinline__ const TraceParams& uniform_params(const BatchSource&, const TraceParams& lane) { return lane; }
+
+template <typename Source, bool kCache>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
                                                               uint32_t regroup, uint32_t refill)
@@ -153,10 +160,11 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                 }
             }
         }
+        const TraceParams& PP = uniform_params(src, P);
         for (uint32_t s = 0; s < regroup; s++) {
             if (S.phase != kIdle) {
-                const StepResult res = (S.phase == kOctant) ? octant_enter(P, S, stack, stride, out)
-                                                            : walk_step(P, S, stack, stride, out);
+                const StepResult res = (S.phase == kOctant) ? octant_enter<kCache>(PP, S, stack, stride, out)
+                                                            : walk_step<kCache>(PP, S, stack, stride, out);
                 if (res == kDone) *dst = out;
             }
         }
@@ -213,15 +221,20 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         const uint64_t blocks = (n + block - 1) / block;
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
         hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
-    } else if (variant == 2) {
+    } else if (variant == 2 || variant == 3) {
         uint32_t blocks, regroup, refill;
         cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill);
         if (st != CQX_OK) return st;
         SingleSource src{ P, rays, hits };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter));
-        hipLaunchKernelGGL(k_intersect_persistent<SingleSource>, dim3(blocks), dim3(block), lds, stream, src, n, n_dev,
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
+        unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
+        if (variant == 2)
+            hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
+                               n, n_dev, counters, regroup, refill);
+        else
+            hipLaunchKernelGGL((k_intersect_persistent<SingleSource, true>), dim3(blocks), dim3(block), lds, stream, src,
+                               n, n_dev, counters, regroup, refill);
     } else {
         return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
     }
@@ -480,8 +493,13 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     BatchSource src{ static_cast<const BatchDesc*>(ctx->batch_desc), n_batches,
                      (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters };
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(ctx->d_counter));
-    hipLaunchKernelGGL(k_intersect_persistent<BatchSource>, dim3(blocks), dim3(block), lds, s, src, total,
-                       (const uint32_t*)nullptr, reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill);
+    unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
+    if (cfg && cfg->variant == 3)
+        hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true>), dim3(blocks), dim3(block), lds, s, src, total,
+                           (const uint32_t*)nullptr, counters, regroup, refill);
+    else
+        hipLaunchKernelGGL((k_intersect_persistent<BatchSource, false>), dim3(blocks), dim3(block), lds, s, src, total,
+                           (const uint32_t*)nullptr, counters, regroup, refill);
     HIP_TRY(hipGetLastError());
     return CQX_OK;
 }

@@ -104,7 +104,27 @@ struct RayState {
     float last_exit;
     uint32_t entries, iters;
     uint32_t phase;
+    uint32_t nc[8]; // the current node's 8 children (node-cache variant only)
 };
+
+// Node cache: the whole 32-byte node is loaded when the walk enters it (SubDAG root, descent,
+// pop), so that sibling advances read the child from registers instead of memory.
+__device__ __forceinline__ void load_node(const TraceParams& P, uint32_t node, uint32_t (&nc)[8])
+{
+    const uint4* p = reinterpret_cast<const uint4*>(P.nodes) + uint64_t(node) * 2;
+    const uint4 a = p[0], b = p[1];
+    nc[0] = a.x; nc[1] = a.y; nc[2] = a.z; nc[3] = a.w;
+    nc[4] = b.x; nc[5] = b.y; nc[6] = b.z; nc[7] = b.w;
+}
+
+// nc[s] for a lane-varying s without dynamic register indexing (a select tree).
+__device__ __forceinline__ uint32_t pick8(const uint32_t (&nc)[8], uint32_t s)
+{
+    const bool b0 = s & 1u, b1 = s & 2u, b2 = s & 4u;
+    const uint32_t a0 = b0 ? nc[1] : nc[0], a1 = b0 ? nc[3] : nc[2], a2 = b0 ? nc[5] : nc[4], a3 = b0 ? nc[7] : nc[6];
+    const uint32_t c0 = b1 ? a1 : a0, c1 = b1 ? a3 : a2;
+    return b2 ? c1 : c0;
+}
 
 __device__ __forceinline__ void result_miss(cqx_hit& out)
 {
@@ -178,6 +198,7 @@ __device__ __forceinline__ StepResult octant_advance(RayState& S, cqx_hit& out)
 
 // Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
 // (:220-250), or move on to the next octant.
+template <bool kCache = false>
 __device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
                                                    cqx_hit& out)
 {
@@ -198,6 +219,7 @@ __device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayStat
             S.height = sd.height;
             for (int h = 0; h <= S.start; h++) stack[h * stride] = 0;
             S.node = sd.node;
+            if (kCache) load_node(P, S.node, S.nc);
             S.csize = int32_t(node_size / 2);
             const uint32_t cid = first_child(entry, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(px, S.csize),
                                              wadd(py, S.csize), wadd(pz, S.csize));
@@ -215,6 +237,7 @@ __device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayStat
 }
 
 // One iteration of the ESVO loop, raytracing.cpp:253-367.
+template <bool kCache = false>
 __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
                                                 cqx_hit& out)
 {
@@ -229,7 +252,7 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
     const float t1z = (float(wadd(S.cpz, csize)) - S.oz) * S.iz;
     const float tcx = min3f(t1x, t1y, t1z);
     const uint32_t slot = S.cid ^ S.sign_bits;
-    const uint32_t child = P.nodes[uint64_t(S.node) * 8 + slot];
+    const uint32_t child = kCache ? pick8(S.nc, slot) : P.nodes[uint64_t(S.node) * 8 + slot];
     if (child > 0) {
         const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
                     t0z = (float(S.cpz) - S.oz) * S.iz;
@@ -244,6 +267,7 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
             S.last_exit = tcx;
             S.height--;
             S.node = child;
+            if (kCache) load_node(P, S.node, S.nc);
             const int32_t half = csize / 2;
             S.csize = half;
             S.entries++;
@@ -315,6 +339,7 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
         S.height = height;
         if (height > S.start) return octant_advance(S, out); // left the SubDAG without a hit
         S.node = stack[height * stride];
+        if (kCache) load_node(P, S.node, S.nc);
         const int32_t ns = int32_t(1u << msb);
         S.csize = ns;
         const uint32_t bx = uint32_t(S.cpx >> msb) & 1u, by = uint32_t(S.cpy >> msb) & 1u,

@@ -222,7 +222,7 @@ cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, u
 
 /* Launch parameters of the traversal kernel. Zero fields take the tuned or built-in defaults. */
 typedef struct {
-    uint32_t variant;          /* 0 auto, 1 one ray per lane, 2 persistent with wave refill */
+    uint32_t variant;          /* 0 auto, 1 one ray per lane, 2 persistent with wave refill, 3 = 2 + node cache */
     uint32_t block;            /* threads per workgroup (multiple of 64) */
     uint32_t waves_per_cu;     /* persistent grid size: CUs x this many waves */
     uint32_t regroup_interval; /* persistent: traversal steps between refill checks */
