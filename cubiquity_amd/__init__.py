@@ -13,7 +13,7 @@ import numpy as np
 
 from . import cabi
 from .cabi import (CQX_SURFACE_PROPERTIES, MAX_FOOTPRINT_DISABLED, CqxError, GenParams, LaunchCfg,
-                   HIT_DTYPE, INTERSECTION_DTYPE, SUBDAG_DTYPE, lib, check)
+                   HIT_DTYPE, INTERSECTION_DTYPE, SPHERE_EDIT_DTYPE, SUBDAG_DTYPE, lib, check)
 
 __all__ = [
     "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
@@ -79,14 +79,26 @@ class HostVolume:
         """Sphere fill/carve edits (fillBrush with SphereBrush semantics). `edits`: (n, 5) array of
         centre x, y, z, radius, material (0 carves). Returns the index of the first appended node."""
         e = np.asarray(edits, dtype=np.float64).reshape(-1, 5)
-        arr = (cabi.SphereEdit * max(1, len(e)))()
-        for i, (x, y, z, r, m) in enumerate(e):
-            arr[i].centre = (ctypes.c_float * 3)(x, y, z)
-            arr[i].radius = r
-            arr[i].material = int(m)
+        arr = np.zeros(max(1, len(e)), dtype=SPHERE_EDIT_DTYPE)
+        arr["centre"][: len(e)] = e[:, :3]
+        arr["radius"][: len(e)] = e[:, 3]
+        if len(e) and not (np.all(e[:, 4] >= 0) and np.all(e[:, 4] < 2**32)):
+            raise CqxError("material out of range")
+        arr["material"][: len(e)] = e[:, 4].astype(np.uint64)
         first = ctypes.c_uint64()
-        check(lib().cqx_hv_edit_spheres(self._h, arr, ctypes.c_uint64(len(e)), ctypes.byref(first)))
+        check(lib().cqx_hv_edit_spheres(self._h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(e)),
+                                        ctypes.byref(first)))
         return int(first.value)
+
+    def checkpoint(self):
+        """Volume::checkpoint: later undo() returns to this state."""
+        check(lib().cqx_hv_checkpoint(self._h))
+
+    def undo(self):
+        check(lib().cqx_hv_undo(self._h))
+
+    def redo(self):
+        check(lib().cqx_hv_redo(self._h))
 
     def compact(self):
         check(lib().cqx_hv_compact(self._h))

@@ -60,6 +60,10 @@ class SphereEdit(ctypes.Structure):
     _fields_ = [("centre", ctypes.c_float * 3), ("radius", ctypes.c_float), ("material", ctypes.c_uint32)]
 
 
+SPHERE_EDIT_DTYPE = np.dtype([("centre", "<f4", (3,)), ("radius", "<f4"), ("material", "<u4")])
+assert SPHERE_EDIT_DTYPE.itemsize == ctypes.sizeof(SphereEdit) == 20
+
+
 class Batch(ctypes.Structure):
     _fields_ = [("volume", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p),
                 ("n", ctypes.c_uint64), ("flags", ctypes.c_uint32), ("max_footprint", ctypes.c_float)]
@@ -95,6 +99,9 @@ SIGNATURES = {
     "cqx_hv_from_dense": (_ST, [_P, _P, _P, _P]),
     "cqx_hv_edit_spheres": (_ST, [_P, _P, _U64, _P]),
     "cqx_hv_compact": (_ST, [_P]),
+    "cqx_hv_checkpoint": (_ST, [_P]),
+    "cqx_hv_undo": (_ST, [_P]),
+    "cqx_hv_redo": (_ST, [_P]),
     "cqx_volume_update": (_ST, [_P, _P, ctypes.POINTER(ctypes.c_uint32), _U64, _U64, _U32, _P]),
     "cqx_camera_rays": (_ST, [_P, _P, ctypes.c_double, _U32, _U32, _P]),
     "cqx_incoherent_rays": (_ST, [_P, _I32, _U64, _U64, ctypes.c_int, _P]),

@@ -26,6 +26,15 @@ static_assert(sizeof(Node) == 32, "node is 32 bytes");
 struct HostVolume {
     std::vector<Node> nodes; // includes the 256 material nodes
     uint32_t root = 0;
+    // Edit history (Volume::checkpoint/undo/redo, storage.cpp:358-385): edits never modify an
+    // existing node, so a checkpoint only needs the root index. roots[current] is brought up to
+    // date from `root` whenever the history is used.
+    std::vector<uint32_t> roots{ 0 };
+    size_t current = 0;
+    // Hash-consing index over `nodes` kept between edit calls (empty until the first edit,
+    // dropped by operations that renumber nodes).
+    std::vector<uint32_t> edit_slots;
+    size_t edit_indexed = 0; // nodes [256, edit_indexed) are in edit_slots
 };
 
 // A NodeStore holding only the 256 material nodes (storage.cpp:110-122).

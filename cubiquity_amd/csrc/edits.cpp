@@ -36,18 +36,25 @@ inline uint64_t hash_node(const Node& n)
     return h;
 }
 
-// Hash-consing index over an existing node vector (built on first use, grown as nodes append).
+// Hash-consing index over a node vector. The slot table lives in the host volume between calls:
+// on construction only the nodes appended since the last call (or all, the first time) are
+// indexed.
 struct Interner {
     std::vector<Node>& nodes;
-    std::vector<uint32_t> slots;
+    std::vector<uint32_t>& slots;
+    size_t& indexed;
     static constexpr uint32_t kEmpty = 0xffffffffu;
 
-    explicit Interner(std::vector<Node>& n) : nodes(n)
+    Interner(std::vector<Node>& n, std::vector<uint32_t>& s, size_t& idx) : nodes(n), slots(s), indexed(idx)
     {
         size_t cap = 1024;
         while (cap < 2 * nodes.size()) cap *= 2;
-        slots.assign(cap, kEmpty);
-        for (size_t j = kMaterialCount; j < nodes.size(); j++) insert_slot(uint32_t(j));
+        if (slots.size() < cap || indexed < kMaterialCount || indexed > nodes.size()) {
+            slots.assign(cap, kEmpty);
+            indexed = kMaterialCount;
+        }
+        for (size_t j = indexed; j < nodes.size(); j++) insert_slot(uint32_t(j));
+        indexed = nodes.size();
     }
 
     void insert_slot(uint32_t j)
@@ -78,6 +85,7 @@ struct Interner {
                 if (nodes.size() >= 0xfffffffeull) throw std::bad_alloc();
                 slots[i] = uint32_t(nodes.size());
                 nodes.push_back(n);
+                indexed = nodes.size();
                 return slots[i];
             }
             if (std::memcmp(&nodes[s], &n, sizeof(Node)) == 0) return s;
@@ -182,7 +190,7 @@ cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits
         if (!(edits[i].radius >= 0.0f)) return fail(CQX_ERR_INVALID_ARG, "edit %llu: bad radius", (unsigned long long)i);
     }
     try {
-        Interner in(hv->v.nodes);
+        Interner in(hv->v.nodes, hv->v.edit_slots, hv->v.edit_indexed);
         uint32_t root = hv->v.root;
         for (uint64_t i = 0; i < n; i++) {
             const SphereBrushF b(edits[i]);
@@ -201,10 +209,46 @@ cqx_status cqx_hv_compact(cqx_host_volume* hv)
     if (!hv) return fail(CQX_ERR_INVALID_ARG, "null argument");
     try {
         canonical_order(hv->v.nodes, hv->v.root);
+        // renumbering invalidates the edit index and the history's other roots
+        hv->v.edit_slots.clear();
+        hv->v.edit_indexed = 0;
+        hv->v.roots.assign(1, hv->v.root);
+        hv->v.current = 0;
         return CQX_OK;
     } catch (const std::bad_alloc&) {
         return fail(CQX_ERR_OOM, "out of memory while compacting");
     }
+}
+
+// Volume::checkpoint, storage.cpp:358-371: later undo() returns to the current state. Redo
+// states beyond the current one are discarded.
+cqx_status cqx_hv_checkpoint(cqx_host_volume* hv)
+{
+    if (!hv) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    hv->v.roots.resize(hv->v.current + 1);
+    hv->v.roots[hv->v.current] = hv->v.root;
+    hv->v.roots.push_back(hv->v.root);
+    hv->v.current++;
+    return CQX_OK;
+}
+
+// Volume::undo / redo, storage.cpp:373-385 (no-ops at either end of the history).
+cqx_status cqx_hv_undo(cqx_host_volume* hv)
+{
+    if (!hv) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    hv->v.roots[hv->v.current] = hv->v.root;
+    if (hv->v.current > 0) hv->v.current--;
+    hv->v.root = hv->v.roots[hv->v.current];
+    return CQX_OK;
+}
+
+cqx_status cqx_hv_redo(cqx_host_volume* hv)
+{
+    if (!hv) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    hv->v.roots[hv->v.current] = hv->v.root;
+    if (hv->v.current + 1 < hv->v.roots.size()) hv->v.current++;
+    hv->v.root = hv->v.roots[hv->v.current];
+    return CQX_OK;
 }
 
 } // extern "C"

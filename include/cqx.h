@@ -178,6 +178,17 @@ cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits
  * storage.cpp:387-394). Invalidates device copies (upload again). */
 cqx_status cqx_hv_compact(cqx_host_volume* hv);
 
+/* Edit history, replacing Volume::checkpoint / undo / redo (storage.cpp:358-385; declared in
+ * storage.h). checkpoint() records the current root; undo() and redo() move along the recorded
+ * roots and are no-ops at either end; a checkpoint after an undo discards the redo states.
+ * Edits append nodes and never modify existing ones, so every recorded state stays valid in the
+ * store and on the device: after undo/redo, cqx_volume_update with first == node_count and the
+ * new root brings a device copy up to date without copying nodes. cqx_hv_compact clears the
+ * history (it renumbers nodes). */
+cqx_status cqx_hv_checkpoint(cqx_host_volume* hv);
+cqx_status cqx_hv_undo(cqx_host_volume* hv);
+cqx_status cqx_hv_redo(cqx_host_volume* hv);
+
 /* Build the canonical (baked) DAG of a dense grid of material ids, x fastest then y then z,
  * whose voxel (0,0,0) sits at signed voxel coordinates `lower` (any placement, including across
  * the origin). The counterpart of filling a Volume with setVoxel and calling bake

@@ -210,6 +210,9 @@ class Reference:
             "ref_fill_sphere": (None, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _U32]),
             "ref_volume_dense": (None, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P]),
             "ref_volume_bake": (None, [_P]),
+            "ref_volume_checkpoint": (None, [_P]),
+            "ref_volume_undo": (None, [_P]),
+            "ref_volume_redo": (None, [_P]),
             "ref_volume_count_nodes": (_U32, [_P]),
             "ref_volume_voxel": (_U32, [_P, _I32, _I32, _I32]),
             "ref_find_subdags": (None, [_P, _P]),
@@ -257,6 +260,15 @@ class RefVolume:
     def fill_sphere(self, centre, radius, material):
         self.ref.L.ref_fill_sphere(self.h, float(centre[0]), float(centre[1]), float(centre[2]), float(radius),
                                    int(material))
+
+    def checkpoint(self):
+        self.ref.L.ref_volume_checkpoint(self.h)
+
+    def undo(self):
+        self.ref.L.ref_volume_undo(self.h)
+
+    def redo(self):
+        self.ref.L.ref_volume_redo(self.h)
 
     def dense(self, lower, size):
         """Voxels of the box [lower, lower + size) as a [z][y][x] uint8 array."""

@@ -74,6 +74,11 @@ void ref_volume_dense(void* h, int32_t x0, int32_t y0, int32_t z0, int32_t nx, i
 
 void ref_volume_bake(void* h) { static_cast<Volume*>(h)->bake(); }
 
+// Volume::checkpoint / undo / redo, storage.cpp:358-385
+void ref_volume_checkpoint(void* h) { static_cast<Volume*>(h)->checkpoint(); }
+void ref_volume_undo(void* h) { static_cast<Volume*>(h)->undo(); }
+void ref_volume_redo(void* h) { static_cast<Volume*>(h)->redo(); }
+
 uint32_t ref_volume_count_nodes(void* h) { return static_cast<Volume*>(h)->countNodes(); }
 
 uint32_t ref_volume_voxel(void* h, int32_t x, int32_t y, int32_t z)

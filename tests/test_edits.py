@@ -97,3 +97,80 @@ def test_gpu_traversal_after_delta_updates(cq, oracle, gpu_ctx):
     img = cq.pathtrace(gpu_ctx, dv, s, 0).cpu().numpy()
     assert np.array_equal(img, oracle.pathtrace(hv.nodes, hv.root, s.params))
     dv.free()
+
+
+def test_checkpoint_undo_redo_match_reference(cq, oracle, reference, tmp_path):
+    """Volume::checkpoint/undo/redo: after every step of an edit session the voxels equal the
+    compiled reference's, including redo states discarded by a checkpoint after an undo."""
+    hv = cq.HostVolume.generate(cq.gen_default_params(6, 11, n_materials=9))
+    path = str(tmp_path / "v.dag")
+    hv.save_dag(path)
+    rv = reference.load(path)
+    lo, size = (-4, -4, -4), (72, 72, 72)
+    script = ["e", "c", "e", "e", "c", "e", "u", "u", "r", "e", "c", "u", "u", "u", "u", "r", "r", "r", "r",
+              "u", "c", "e", "r", "u", "e"]
+    states = []
+    for i, op in enumerate(script):
+        if op == "e":
+            e = _edits(300 + i, 12)
+            hv.edit_spheres(e)
+            for x in e:
+                rv.fill_sphere(x[:3], x[3], int(x[4]))
+        elif op == "c":
+            hv.checkpoint()
+            rv.checkpoint()
+        elif op == "u":
+            hv.undo()
+            rv.undo()
+        else:
+            hv.redo()
+            rv.redo()
+        ours = oracle.dense(hv.nodes, hv.root, lo, size)
+        assert np.array_equal(ours, rv.dense(lo, size)), f"step {i} ({op})"
+        states.append(ours.tobytes())
+    assert len(set(states)) >= 5  # the script visits distinct states
+
+
+def test_undo_keeps_earlier_nodes_valid(cq, oracle):
+    """Edits never modify existing nodes, so a root recorded by a checkpoint stays a valid DAG and
+    an undo needs no node copies."""
+    hv = cq.HostVolume.generate(cq.gen_default_params(6, 12))
+    r0 = hv.root
+    dense0 = oracle.dense(hv.nodes, r0, (0, 0, 0), (64, 64, 64))
+    hv.checkpoint()
+    n_before = hv.node_count
+    hv.edit_spheres(_edits(7, 30))
+    assert hv.node_count > n_before
+    hv.undo()
+    assert hv.root == r0
+    assert np.array_equal(oracle.dense(hv.nodes, hv.root, (0, 0, 0), (64, 64, 64)), dense0)
+    hv.redo()
+    assert hv.root != r0
+    hv.undo()
+    hv.undo()  # no-op at the start of the history
+    assert hv.root == r0
+    hv.compact()  # renumbers: the history is reset to the current state
+    hv.redo()
+    assert np.array_equal(oracle.dense(hv.nodes, hv.root, (0, 0, 0), (64, 64, 64)), dense0)
+
+
+@pytest.mark.gpu
+def test_gpu_traversal_after_undo_redo(cq, oracle, gpu_ctx):
+    """Root-only device updates after undo/redo: traversal bit-exact against the oracle."""
+    import torch
+
+    hv = cq.HostVolume.generate(cq.gen_default_params(7, 13, n_materials=9))
+    dv = gpu_ctx.upload(hv)
+    rays = cq.incoherent_rays(hv, 7, 9, 20000)
+    r = torch.from_numpy(rays).cuda()
+    hv.checkpoint()
+    dv.update(hv, hv.edit_spheres(_edits(41, 300, -4.0, 132.0)))
+    hv.checkpoint()
+    dv.update(hv, hv.edit_spheres(_edits(42, 300, -4.0, 132.0)))
+    for op in ("undo", "undo", "redo", "redo", "undo"):
+        getattr(hv, op)()
+        dv.update(hv, hv.node_count)  # no node copies, root and SubDAGs only
+        got = cq.intersectVolume(gpu_ctx, dv, r, True).cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
+        ref = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, threads=8)
+        assert records_equal(got, ref).all(), op
+    dv.free()
