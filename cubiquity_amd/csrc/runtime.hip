@@ -2,11 +2,13 @@
 //
 // Kernels
 //   k_intersect_lane        one ray per lane, one pass over the ray array (variant 1)
-//   k_intersect_persistent  persistent waves that keep every lane busy: a lane whose ray is done
-//                           stores its record and, once enough lanes of the wave are idle,
-//                           the wave claims a contiguous block of new rays from a global counter
-//                           with one atomic and hands them out by ballot prefix (variant 2);
-//                           variant 3 also keeps the current node's 8 children in registers
+//   k_intersect_persistent  persistent waves whose traversal steps run SubDAG walks only: a lane
+//                           whose walk ends waits; once enough lanes of the wave wait, a service
+//                           pass moves them on to their next SubDAG, writes the records of the
+//                           finished rays, and claims a contiguous block of new rays from a
+//                           global counter with one atomic, handed out by ballot prefix
+//                           (variant 2); variant 3 also keeps the current node's 8 children in
+//                           registers
 // All keep the traversal stack in LDS (one column per lane, one row per node height).
 #include "device_internal.h"
 
@@ -113,7 +115,7 @@ __device__ __forceinline__ const TraceParams& uniform_params(const BatchSource&,
 template <typename Source, bool kCache>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
-                                                              uint32_t regroup, uint32_t refill)
+                                                              uint32_t regroup, uint32_t refill, uint32_t rows)
 {
     extern __shared__ uint32_t lds_stack[];
     if (n_dev) {
@@ -123,51 +125,61 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
     }
     TraceParams P;
     cqx_hit* dst = nullptr;
-    uint32_t* stack = lds_stack + threadIdx.x;
-    const uint32_t stride = blockDim.x;
+    // Each wave owns `rows` x 64 words of the block's stack memory; slot h of a lane is
+    // stack[h * 64] (a shift, not a multiply, in the address).
     const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* stack = lds_stack + (threadIdx.x >> 6) * rows * 64u + lane;
+    constexpr uint32_t stride = 64u;
     const uint64_t per_q = (n + kQueues - 1) / kQueues;
     uint32_t qc = blockIdx.x % kQueues; // current queue
     uint32_t drained = 0;               // queues found empty so far
 
     RayState S;
     S.phase = kIdle;
-    cqx_hit out;
     bool exhausted = false;
 
     for (;;) {
-        const uint64_t idle = __ballot(S.phase == kIdle);
-        const uint32_t nidle = uint32_t(__popcll(idle));
-        if (exhausted && nidle == 64) break;
-        if (!exhausted && (nidle >= refill || nidle == 64)) {
-            const uint64_t q_begin = per_q * qc;
-            const uint64_t q_end = q_begin + per_q < n ? q_begin + per_q : n;
-            const uint64_t q_len = q_end > q_begin ? q_end - q_begin : 0;
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(&counters[qc], (unsigned long long)nidle);
-            base = __shfl(base, 0);
-            if (base + nidle >= q_len) {
-                qc = (qc + 1) % kQueues;
-                if (++drained == kQueues) exhausted = true;
+        // Lanes are busy while they walk a SubDAG. Between SubDAGs, finished or idle, they wait
+        // for the next service pass, which runs every `regroup` steps once `refill` lanes wait.
+        const uint64_t walking = __ballot(S.phase == kWalk);
+        const uint32_t nwait = 64u - uint32_t(__popcll(walking));
+        if (nwait >= refill || walking == 0) {
+            enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
+            if (is_finished(S.phase)) {
+                cqx_hit out;
+                emit_record(uniform_params(src, P), S, out);
+                *dst = out;
+                S.phase = kIdle;
             }
-            if (S.phase == kIdle) {
-                const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                const uint64_t k = base + rank;
-                if (k < q_len) {
-                    const float* r;
-                    src.fetch(q_begin + k, P, r, dst);
-                    ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
+            if (!exhausted) {
+                const uint64_t idle = __ballot(S.phase == kIdle);
+                const uint32_t nidle = uint32_t(__popcll(idle));
+                const uint64_t q_begin = per_q * qc;
+                const uint64_t q_end = q_begin + per_q < n ? q_begin + per_q : n;
+                const uint64_t q_len = q_end > q_begin ? q_end - q_begin : 0;
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&counters[qc], (unsigned long long)nidle);
+                base = __shfl(base, 0);
+                if (base + nidle >= q_len) {
+                    qc = (qc + 1) % kQueues;
+                    if (++drained == kQueues) exhausted = true;
+                }
+                if (S.phase == kIdle) {
+                    const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                    const uint64_t k = base + rank;
+                    if (k < q_len) {
+                        const float* r;
+                        src.fetch(q_begin + k, P, r, dst);
+                        ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
+                        enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
+                    }
                 }
             }
+            if (exhausted && __ballot(S.phase != kIdle) == 0) break;
         }
         const TraceParams& PP = uniform_params(src, P);
-        for (uint32_t s = 0; s < regroup; s++) {
-            if (S.phase != kIdle) {
-                const StepResult res = (S.phase == kOctant) ? octant_enter<kCache>(PP, S, stack, stride, out)
-                                                            : walk_step<kCache>(PP, S, stack, stride, out);
-                if (res == kDone) *dst = out;
-            }
-        }
+        for (uint32_t s = 0; s < regroup; s++)
+            if (S.phase == kWalk) walk_step<kCache>(PP, S, stack, stride);
     }
 }
 
@@ -231,10 +243,10 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
         if (variant == 2)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill);
+                               n, n_dev, counters, regroup, refill, rows);
         else
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, true>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill);
+                               n, n_dev, counters, regroup, refill, rows);
     } else {
         return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
     }
@@ -473,7 +485,8 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     if (total > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "more than 2^38 rays in one launch");
     const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
     if (block % 64 != 0 || block == 0 || block > 256) return fail(CQX_ERR_INVALID_ARG, "bad block size %u", block);
-    const size_t lds = size_t(max_height + 1) * block * sizeof(uint32_t);
+    const uint32_t rows = uint32_t(max_height) + 1;
+    const size_t lds = size_t(rows) * block * sizeof(uint32_t);
     if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
     hipStream_t s = hipStream_t(stream_ptr);
     // The descriptor table is per context: wait for an earlier batched launch on this stream.
@@ -496,10 +509,10 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
     if (cfg && cfg->variant == 3)
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows);
     else
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, false>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows);
     HIP_TRY(hipGetLastError());
     return CQX_OK;
 }

@@ -12,6 +12,12 @@
 //   octant_enter   SubDAG of the current octant: slab test and ESVO initialisation (:443-458, :220-250)
 //   walk_step      one iteration of the ESVO do/while loop (:253-367)
 //   octant_advance next octant in order of axis crossings (:469-475)
+//   emit_record    the result record of a finished ray (hit epilogue :302-320, findNearestMaterial)
+// A step that ends the walk of a SubDAG only records the outcome (phase kLeave, or kHit / kMiss /
+// kAnomaly with the hit child in `start`). Moving on to the next SubDAG (enter_octants) and
+// building the record (emit_record, which recomputes every value it needs from the state with the
+// walk's own operations) happen later, so that a persistent wave runs that code for many lanes at
+// once instead of in almost every traversal step.
 // The traversal stack lives in LDS, one column per lane: slot h of a lane is stack[h * stride].
 #pragma once
 
@@ -87,8 +93,11 @@ __device__ __forceinline__ bool footprint_large(const TraceParams& P, int32_t cs
     return (float(csize) / tcx) > P.max_footprint;
 }
 
-enum : uint32_t { kIdle = 0, kOctant = 1, kWalk = 2 };
-enum StepResult : uint32_t { kContinue = 0, kDone = 1 };
+// kLeave: the walk left the current SubDAG without a hit; the next octant is still to be found.
+enum : uint32_t { kIdle = 0, kOctant = 1, kWalk = 2, kLeave = 3, kHit = 4, kMiss = 5, kAnomaly = 6 };
+
+// Phases of a lane whose walk is over and whose record is not yet written.
+__device__ __forceinline__ bool is_finished(uint32_t phase) { return phase >= kHit; }
 
 struct RayState {
     float ox, oy, oz;       // reflected, shifted origin
@@ -98,7 +107,7 @@ struct RayState {
     uint32_t sign_bits;
     uint32_t oct;
     uint32_t node;
-    int32_t height, start, csize;
+    int32_t height, start, csize; // start: the hit child once the phase is kHit
     uint32_t cid;
     int32_t cpx, cpy, cpz;
     float last_exit;
@@ -134,17 +143,6 @@ __device__ __forceinline__ void result_miss(cqx_hit& out)
     out.info = 0;
 }
 
-__device__ __forceinline__ void finish(RayState& S, cqx_hit& out, bool anomaly)
-{
-    if (anomaly) {
-        result_miss(out);
-        out.info = CQX_HIT_ANOMALY;
-    }
-    out.entries = S.entries;
-    out.iters = S.iters;
-    S.phase = kIdle;
-}
-
 // Ray setup, raytracing.cpp:415-434.
 __device__ __forceinline__ void ray_setup(RayState& S, float rox, float roy, float roz, float rdx, float rdy, float rdz)
 {
@@ -173,34 +171,23 @@ __device__ __forceinline__ void ray_setup(RayState& S, float rox, float roy, flo
     S.phase = kOctant;
 }
 
-// Next octant in order of axis crossings, raytracing.cpp:469-475. Returns kDone when the walk is
-// over (all octants visited: a miss; or no axis steps: the reference would loop forever).
-__device__ __forceinline__ StepResult octant_advance(RayState& S, cqx_hit& out)
+// Next octant in order of axis crossings, raytracing.cpp:469-475. The walk is over when all
+// octants were visited (a miss) or when no axis steps (the reference would loop forever).
+__device__ __forceinline__ void octant_advance(RayState& S)
 {
     const float m = min3f(S.distx, S.disty, S.distz);
     uint32_t step = 0;
     if (S.distx <= m) { step += 1; S.distx += kFltMax; }
     if (S.disty <= m) { step += 2; S.disty += kFltMax; }
     if (S.distz <= m) { step += 4; S.distz += kFltMax; }
-    if (step == 0) {
-        finish(S, out, true);
-        return kDone;
-    }
     S.oct += step;
-    if (S.oct > 7) {
-        result_miss(out);
-        finish(S, out, false);
-        return kDone;
-    }
-    S.phase = kOctant;
-    return kContinue;
+    S.phase = step == 0 ? kAnomaly : (S.oct > 7 ? kMiss : kOctant);
 }
 
 // Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
 // (:220-250), or move on to the next octant.
 template <bool kCache = false>
-__device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                                   cqx_hit& out)
+__device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
 {
     const cqx_subdag sd = P.sub[S.oct ^ S.sign_bits];
     if (sd.node > 0) {
@@ -230,20 +217,19 @@ __device__ __forceinline__ StepResult octant_enter(const TraceParams& P, RayStat
             S.last_exit = exitt;
             S.entries++;
             S.phase = kWalk;
-            return kContinue;
+            return;
         }
     }
-    return octant_advance(S, out);
+    octant_advance(S);
 }
 
 // One iteration of the ESVO loop, raytracing.cpp:253-367.
 template <bool kCache = false>
-__device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                                cqx_hit& out)
+__device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
 {
     if (S.iters >= P.max_iters) {
-        finish(S, out, true);
-        return kDone;
+        S.phase = kAnomaly;
+        return;
     }
     S.iters++;
     const int32_t csize = S.csize;
@@ -254,21 +240,21 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
     const uint32_t slot = S.cid ^ S.sign_bits;
     const uint32_t child = kCache ? pick8(S.nc, slot) : P.nodes[uint64_t(S.node) * 8 + slot];
     if (child > 0) {
-        const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
-                    t0z = (float(S.cpz) - S.oz) * S.iz;
-        const float tce = max3f(t0x, t0y, t0z);
         const bool internal = child >= kMaterialCount;
         if (internal && footprint_large(P, csize, tcx)) {
             if (S.height <= 1) {
-                finish(S, out, true); // deeper than a valid DAG
-                return kDone;
+                S.phase = kAnomaly; // deeper than a valid DAG
+                return;
             }
+            const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
+                        t0z = (float(S.cpz) - S.oz) * S.iz;
+            const float tce = max3f(t0x, t0y, t0z);
             if (tcx < S.last_exit) stack[S.height * stride] = S.node;
             S.last_exit = tcx;
             S.height--;
             S.node = child;
             if (kCache) load_node(P, S.node, S.nc);
-            const int32_t half = csize / 2;
+            const int32_t half = int32_t(uint32_t(csize) >> 1); // csize = 2^(height - 1) >= 2 here
             S.csize = half;
             S.entries++;
             const uint32_t cid = first_child(tce, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(S.cpx, half),
@@ -277,51 +263,12 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
             S.cpx = wadd(S.cpx, (cid & 1u) ? half : 0);
             S.cpy = wadd(S.cpy, (cid & 2u) ? half : 0);
             S.cpz = wadd(S.cpz, (cid & 4u) ? half : 0);
-            return kContinue;
+            return;
         }
-        // Hit (raytracing.cpp:302-320). A SubDAG of height 0 (the chain below the root ends in a
-        // single voxel) has child size 0: the hit block is then the SubDAG node itself, of side 1.
-        const int32_t bsize = csize > 0 ? csize : 1;
-        uint32_t info = 1u | (slot << 1) | (uint32_t(find_msb(uint32_t(bsize))) << 4);
-        if (P.surf) {
-            // findNearestMaterial, raytracing.cpp:134-162
-            const uint32_t order = (0x76534210u | 0x88888888u) ^ (S.sign_bits * 0x11111111u);
-            uint32_t m = child;
-            while (m >= kMaterialCount) {
-                S.entries++;
-                uint32_t next = 0;
-                for (uint32_t ids = order; ids != 0; ids >>= 4) {
-                    const uint32_t c = P.nodes[uint64_t(m) * 8 + (ids & 7u)];
-                    if (c > 0) { next = c; break; }
-                }
-                // An internal node without occupied children (never produced by bake) would make
-                // the reference loop forever.
-                if (next == 0) {
-                    finish(S, out, true);
-                    return kDone;
-                }
-                m = next;
-            }
-            // normal = vec3(equal(tEntry, childT0)) * -rayDirSign, raytracing.cpp:317-318
-            const float sx = (S.sign_bits & 1u) ? -1.0f : 1.0f, sy = (S.sign_bits & 2u) ? -1.0f : 1.0f,
-                        sz = (S.sign_bits & 4u) ? -1.0f : 1.0f;
-            const float nx = float(tce == t0x) * -sx, ny = float(tce == t0y) * -sy, nz = float(tce == t0z) * -sz;
-            info |= ((nx != 0.0f ? 1u : 0u) | ((__float_as_uint(nx) >> 31) << 1)) << 9;
-            info |= ((ny != 0.0f ? 1u : 0u) | ((__float_as_uint(ny) >> 31) << 1)) << 11;
-            info |= ((nz != 0.0f ? 1u : 0u) | ((__float_as_uint(nz) >> 31) << 1)) << 13;
-            info |= (m & 0xffu) << 16;
-        }
-        const int32_t vx = entered_voxel(S.ox, S.dx, tce, S.cpx, bsize, t0x == tce);
-        const int32_t vy = entered_voxel(S.oy, S.dy, tce, S.cpy, bsize, t0y == tce);
-        const int32_t vz = entered_voxel(S.oz, S.dz, tce, S.cpz, bsize, t0z == tce);
-        out.voxel[0] = (S.sign_bits & 1u) ? ~vx : vx;
-        out.voxel[1] = (S.sign_bits & 2u) ? ~vy : vy;
-        out.voxel[2] = (S.sign_bits & 4u) ? ~vz : vz;
-        out.t = tce;
-        out.node = S.node;
-        out.info = info;
-        finish(S, out, false);
-        return kDone;
+        // Hit (raytracing.cpp:302-320): the record is built by emit_record.
+        S.start = int32_t(child);
+        S.phase = kHit;
+        return;
     }
     // Advance to the next sibling, raytracing.cpp:328-332.
     const uint32_t flips = (t1x <= tcx ? 1u : 0u) | (t1y <= tcx ? 2u : 0u) | (t1z <= tcx ? 4u : 0u);
@@ -337,7 +284,10 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
         const int msb = find_msb(diff);
         const int height = msb + 1;
         S.height = height;
-        if (height > S.start) return octant_advance(S, out); // left the SubDAG without a hit
+        if (height > S.start) { // left the SubDAG without a hit
+            S.phase = kLeave;
+            return;
+        }
         S.node = stack[height * stride];
         if (kCache) load_node(P, S.node, S.nc);
         const int32_t ns = int32_t(1u << msb);
@@ -351,7 +301,87 @@ __device__ __forceinline__ StepResult walk_step(const TraceParams& P, RayState& 
         S.last_exit = 0.0f;
         S.entries++;
     }
-    return kContinue;
+}
+
+// Moves a lane that is between SubDAGs (phase kLeave or kOctant) into the walk of the next SubDAG
+// the ray enters, or to the end of its walk. At most 8 octants are tried.
+template <bool kCache = false>
+__device__ __forceinline__ void enter_octants(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+{
+    if (S.phase == kLeave) octant_advance(S);
+    while (S.phase == kOctant) octant_enter<kCache>(P, S, stack, stride);
+}
+
+// One traversal step of a lane whose walk is in progress.
+template <bool kCache = false>
+__device__ __forceinline__ void step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+{
+    if (S.phase == kWalk) walk_step<kCache>(P, S, stack, stride);
+    else enter_octants<kCache>(P, S, stack, stride);
+}
+
+// Record of a finished ray (phase kHit, kMiss or kAnomaly). For a hit, the child-box entry
+// times are recomputed from the state with the walk's own operations, so they are the values
+// the hit test saw.
+__device__ __forceinline__ void emit_record(const TraceParams& P, RayState& S, cqx_hit& out)
+{
+    result_miss(out);
+    out.entries = S.entries;
+    out.iters = S.iters;
+    if (S.phase == kAnomaly) {
+        out.info = CQX_HIT_ANOMALY;
+        return;
+    }
+    if (S.phase != kHit) return;
+    const uint32_t child = uint32_t(S.start);
+    const uint32_t slot = S.cid ^ S.sign_bits;
+    const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
+                t0z = (float(S.cpz) - S.oz) * S.iz;
+    const float tce = max3f(t0x, t0y, t0z);
+    // A SubDAG of height 0 (the chain below the root ends in a single voxel) has child size 0:
+    // the hit block is then the SubDAG node itself, of side 1.
+    const int32_t bsize = S.csize > 0 ? S.csize : 1;
+    uint32_t info = 1u | (slot << 1) | (uint32_t(find_msb(uint32_t(bsize))) << 4);
+    if (P.surf) {
+        // findNearestMaterial, raytracing.cpp:134-162
+        const uint32_t order = (0x76534210u | 0x88888888u) ^ (S.sign_bits * 0x11111111u);
+        uint32_t m = child;
+        uint32_t entries = S.entries;
+        while (m >= kMaterialCount) {
+            entries++;
+            uint32_t next = 0;
+            for (uint32_t ids = order; ids != 0; ids >>= 4) {
+                const uint32_t c = P.nodes[uint64_t(m) * 8 + (ids & 7u)];
+                if (c > 0) { next = c; break; }
+            }
+            // An internal node without occupied children (never produced by bake) would make
+            // the reference loop forever.
+            if (next == 0) {
+                out.entries = entries;
+                out.info = CQX_HIT_ANOMALY;
+                return;
+            }
+            m = next;
+        }
+        out.entries = entries;
+        // normal = vec3(equal(tEntry, childT0)) * -rayDirSign, raytracing.cpp:317-318
+        const float sx = (S.sign_bits & 1u) ? -1.0f : 1.0f, sy = (S.sign_bits & 2u) ? -1.0f : 1.0f,
+                    sz = (S.sign_bits & 4u) ? -1.0f : 1.0f;
+        const float nx = float(tce == t0x) * -sx, ny = float(tce == t0y) * -sy, nz = float(tce == t0z) * -sz;
+        info |= ((nx != 0.0f ? 1u : 0u) | ((__float_as_uint(nx) >> 31) << 1)) << 9;
+        info |= ((ny != 0.0f ? 1u : 0u) | ((__float_as_uint(ny) >> 31) << 1)) << 11;
+        info |= ((nz != 0.0f ? 1u : 0u) | ((__float_as_uint(nz) >> 31) << 1)) << 13;
+        info |= (m & 0xffu) << 16;
+    }
+    const int32_t vx = entered_voxel(S.ox, S.dx, tce, S.cpx, bsize, t0x == tce);
+    const int32_t vy = entered_voxel(S.oy, S.dy, tce, S.cpy, bsize, t0y == tce);
+    const int32_t vz = entered_voxel(S.oz, S.dz, tce, S.cpz, bsize, t0z == tce);
+    out.voxel[0] = (S.sign_bits & 1u) ? ~vx : vx;
+    out.voxel[1] = (S.sign_bits & 2u) ? ~vy : vy;
+    out.voxel[2] = (S.sign_bits & 4u) ? ~vz : vz;
+    out.t = tce;
+    out.node = S.node;
+    out.info = info;
 }
 
 // Runs one ray to completion (state machine driven in a loop).
@@ -360,11 +390,8 @@ __device__ __forceinline__ void trace_ray(const TraceParams& P, float rox, float
 {
     RayState S;
     ray_setup(S, rox, roy, roz, rdx, rdy, rdz);
-    for (;;) {
-        const StepResult r = (S.phase == kOctant) ? octant_enter(P, S, stack, stride, out)
-                                                  : walk_step(P, S, stack, stride, out);
-        if (r == kDone) break;
-    }
+    while (!is_finished(S.phase)) step(P, S, stack, stride);
+    emit_record(P, S, out);
 }
 
 } // namespace cqxd
