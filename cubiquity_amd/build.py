@@ -83,13 +83,19 @@ def build(verbose=False):
 
 
 def build_cpp_checks(verbose=False):
-    """The C++ drop-in example run by the GPU tests (links libcqx.so through its rpath)."""
-    src = os.path.join(ROOT, "tests", "cpp", "dropin_check.cpp")
+    """Test programs: the C++ drop-in example run by the GPU tests (links libcqx.so through its
+    rpath) and the host compilation of the device traversal core used by the CPU tests."""
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    src = os.path.join(cpp, "dropin_check.cpp")
     if not os.path.exists(src):
         return None
-    out = os.path.join(ROOT, "tests", "cpp", "dropin_check")
+    out = os.path.join(cpp, "dropin_check")
     _run([CXX, "-std=c++17", "-O2", f"-I{INCLUDE}", src, "-o", out, f"-L{HERE}", "-lcqx",
           "-Wl,-rpath,$ORIGIN/../../cubiquity_amd"], verbose)
+    emu = os.path.join(cpp, "core_emulation.cpp")
+    if os.path.exists(emu):
+        _run([CXX, "-std=c++17", "-O2", "-ffp-contract=off", "-fno-fast-math", f"-I{os.path.join(cpp, 'host_shim')}",
+              f"-I{INCLUDE}", f"-I{CSRC}", emu, "-o", os.path.join(cpp, "core_emulation")], verbose)
     return out
 
 

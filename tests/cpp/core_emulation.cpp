@@ -1,0 +1,100 @@
+// Test infrastructure: the device traversal core (cubiquity_amd/csrc/traverse_core.h) compiled
+// for the host, driven the way the kernels drive it, so that its per-lane logic can be checked
+// against the oracle on the CPU before any GPU run:
+//   lane        trace_ray per ray (k_intersect_lane, the path-tracer kernels)
+//   persistent  64-lane waves with the persistent kernel's service-pass policy: walk steps for
+//               walking lanes only; every `regroup` steps, once `refill` lanes wait (or none
+//               walks), waiting lanes move on to their next SubDAG, finished lanes emit their
+//               record, idle lanes take new rays (k_intersect_persistent)
+// Usage: core_emulation lane|persistent nodes.bin sub.bin rays.bin out.bin surf max_footprint
+//                       max_iters refill regroup
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __restrict__
+#include "traverse_core.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <string_view>
+#include <vector>
+
+using namespace cqxd;
+
+static std::vector<char> slurp(const char* path)
+{
+    std::vector<char> v;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return v;
+    std::fseek(f, 0, SEEK_END);
+    v.resize(size_t(std::ftell(f)));
+    std::fseek(f, 0, SEEK_SET);
+    if (std::fread(v.data(), 1, v.size(), f) != v.size()) v.clear();
+    std::fclose(f);
+    return v;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 11) {
+        std::fprintf(stderr, "usage: %s lane|persistent nodes sub rays out surf fp max_iters refill regroup\n", argv[0]);
+        return 2;
+    }
+    const bool persistent = std::string_view(argv[1]) == "persistent";
+    const std::vector<char> nb = slurp(argv[2]), sb = slurp(argv[3]), rb = slurp(argv[4]);
+    if (nb.empty() || sb.size() != 8 * sizeof(cqx_subdag)) return 3;
+    TraceParams P;
+    P.nodes = reinterpret_cast<const uint32_t*>(nb.data());
+    P.sub = reinterpret_cast<const cqx_subdag*>(sb.data());
+    P.surf = uint32_t(std::atoi(argv[6]));
+    P.max_footprint = float(std::atof(argv[7]));
+    P.max_iters = uint32_t(std::strtoul(argv[8], nullptr, 10));
+    P.fp_disabled = footprint_is_disabled(P.max_footprint) ? 1u : 0u;
+    const int refill = std::atoi(argv[9]), regroup = std::atoi(argv[10]);
+    const float* rays = reinterpret_cast<const float*>(rb.data());
+    const uint64_t n = rb.size() / 24;
+    std::vector<cqx_hit> out(n);
+    std::vector<uint32_t> stack(64 * 33);
+
+    if (!persistent) {
+        for (uint64_t i = 0; i < n; i++) {
+            const float* r = rays + 6 * i;
+            trace_ray(P, r[0], r[1], r[2], r[3], r[4], r[5], stack.data(), 1, out[i]);
+        }
+    } else {
+        RayState S[64];
+        uint64_t idx[64] = {};
+        for (RayState& s : S) s.phase = kIdle;
+        uint64_t next = 0;
+        for (;;) {
+            int walking = 0;
+            for (const RayState& s : S) walking += s.phase == kWalk;
+            if (64 - walking >= refill || walking == 0) {
+                for (int l = 0; l < 64; l++) enter_octants(P, S[l], &stack[l], 64);
+                for (int l = 0; l < 64; l++)
+                    if (is_finished(S[l].phase)) {
+                        emit_record(P, S[l], out[idx[l]]);
+                        S[l].phase = kIdle;
+                    }
+                for (int l = 0; l < 64; l++)
+                    if (S[l].phase == kIdle && next < n) {
+                        const float* r = rays + 6 * next;
+                        idx[l] = next++;
+                        ray_setup(S[l], r[0], r[1], r[2], r[3], r[4], r[5]);
+                        enter_octants(P, S[l], &stack[l], 64);
+                    }
+                int live = 0;
+                for (const RayState& s : S) live += s.phase != kIdle;
+                if (next >= n && live == 0) break;
+            }
+            for (int s = 0; s < regroup; s++)
+                for (int l = 0; l < 64; l++)
+                    if (S[l].phase == kWalk) walk_step(P, S[l], &stack[l], 64);
+        }
+    }
+    FILE* f = std::fopen(argv[5], "wb");
+    if (!f) return 4;
+    const size_t w = std::fwrite(out.data(), sizeof(cqx_hit), n, f);
+    std::fclose(f);
+    return w == n ? 0 : 5;
+}

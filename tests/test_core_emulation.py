@@ -1,0 +1,71 @@
+"""The device traversal core (cubiquity_amd/csrc/traverse_core.h) compiled for the host and driven
+the way the kernels drive it (tests/cpp/core_emulation.cpp): per-ray trace_ray, and 64-lane waves
+with the persistent kernel's service-pass policy (deferred SubDAG changes and records, refills
+between traversal steps) at several refill / regroup settings. Every 32-byte record must be
+bit-identical to the oracle's, so a logic change to the core is caught on the CPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import records_equal
+
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "core_emulation")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not os.path.exists(EXE):
+        from cubiquity_amd import build
+
+        build.build_cpp_checks()
+    assert os.path.exists(EXE)
+    return EXE
+
+
+def _run(emu, tmp_path, cq, hv, rays, mode, surf, fp, max_iters, refill=32, regroup=8):
+    sub = cq.findSubDAGs(hv.nodes, hv.root)
+    paths = [str(tmp_path / f) for f in ("n.bin", "s.bin", "r.bin", "o.bin")]
+    np.ascontiguousarray(hv.nodes).tofile(paths[0])
+    np.ascontiguousarray(sub).tofile(paths[1])
+    np.ascontiguousarray(rays, dtype=np.float32).tofile(paths[2])
+    subprocess.run([emu, mode, *paths, str(int(surf)), repr(float(fp)), str(max_iters), str(refill), str(regroup)],
+                   check=True, timeout=300)
+    return np.fromfile(paths[3], dtype=cq.HIT_DTYPE)
+
+
+@pytest.mark.parametrize("mode,refill,regroup", [("lane", 0, 0), ("persistent", 32, 8), ("persistent", 8, 1),
+                                                 ("persistent", 64, 3)])
+def test_core_matches_oracle_128(emu, tmp_path, cq, oracle, vol128, rays128, mode, refill, regroup):
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    rays = np.concatenate([rays128["camera"][::5], rays128["incoherent"], rays128["special"]])
+    for surf, fp in ((True, -1.0), (False, -1.0), (True, 0.05), (True, 0.0035)):
+        got = _run(emu, tmp_path, cq, vol128, rays, mode, surf, fp, 1 << 16, refill, regroup)
+        ref = oracle.intersect(vol128.nodes, sub, rays, surf, fp, max_iters=1 << 16, threads=8)
+        eq = records_equal(got, ref)
+        assert eq.all(), f"{mode} surf={surf} fp={fp}: {(~eq).sum()} records differ, first {np.nonzero(~eq)[0][:4]}"
+
+
+def test_core_iteration_cap(emu, tmp_path, cq, oracle, vol128, rays128):
+    rays = rays128["camera"][30720:34816]
+    got = _run(emu, tmp_path, cq, vol128, rays, "persistent", True, -1.0, 5)
+    ref = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True, -1.0, max_iters=5)
+    assert ((got["info"] >> 15) & 1).sum() > 0
+    assert records_equal(got, ref).all()
+
+
+def test_core_small_and_degenerate_volumes(emu, tmp_path, cq, oracle):
+    from tests.raysets import special_rays
+    from tests.test_gpu_parity import _small_volumes
+    from tests.test_oracle_pinning import _brute_rays
+
+    for name, (hv, n, lower) in _small_volumes(cq).items():
+        sp = special_rays(n)
+        sp[:, :3] += np.asarray(lower, np.float32)
+        rays = np.concatenate([_brute_rays(n, lower, 3000, 17), sp]).astype(np.float32)
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        for mode in ("lane", "persistent"):
+            got = _run(emu, tmp_path, cq, hv, rays, mode, True, -1.0, 1 << 16, 16, 4)
+            ref = oracle.intersect(hv.nodes, sub, rays, True, -1.0, max_iters=1 << 16)
+            assert records_equal(got, ref).all(), f"{name} {mode}"
