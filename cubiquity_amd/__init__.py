@@ -13,9 +13,10 @@ import numpy as np
 
 from . import cabi
 from .cabi import (CQX_SURFACE_PROPERTIES, MAX_FOOTPRINT_DISABLED, CqxError, GenParams, LaunchCfg,
-                   HIT_DTYPE, INTERSECTION_DTYPE, SPHERE_EDIT_DTYPE, SUBDAG_DTYPE, lib, check)
+                   HIT_DTYPE, INTERSECTION_DTYPE, MATERIAL_DTYPE, SPHERE_EDIT_DTYPE, SUBDAG_DTYPE, lib, check)
 
 __all__ = [
+    "Materials",
     "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
     "incoherent_rays", "gen_default_params", "hits_to_intersections", "CqxError", "GenParams",
     "LaunchCfg", "HIT_DTYPE", "INTERSECTION_DTYPE", "SUBDAG_DTYPE", "CQX_SURFACE_PROPERTIES",
@@ -316,6 +317,61 @@ def intersect_batched(ctx, items, cfg=None, stream=None):
         stream = torch.cuda.current_stream().cuda_stream
     cfgp = ctypes.byref(cfg) if cfg is not None else None
     check(lib().cqx_intersect_batched(ctx._h, arr, len(items), cfgp, ctypes.c_void_p(stream)))
+
+
+class Materials:
+    """Material metadata (the .toml next to a .dag; Metadata, src/application/base/metadata.h:37-51):
+    a structured array of MATERIAL_DTYPE rows indexed by material id, plus optional dimensions."""
+
+    def __init__(self, rows=None, dimensions=None):
+        self.rows = np.zeros(0, MATERIAL_DTYPE) if rows is None else np.asarray(rows, MATERIAL_DTYPE)
+        self.dimensions = dimensions
+
+    @staticmethod
+    def load(path):
+        rows = np.zeros(256, MATERIAL_DTYPE)
+        count = ctypes.c_uint32()
+        dims = (ctypes.c_int32 * 3)()
+        has = ctypes.c_uint32()
+        check(lib().cqx_materials_load(os.fsencode(path), rows.ctypes.data_as(ctypes.c_void_p), ctypes.byref(count),
+                                       dims, ctypes.byref(has)))
+        return Materials(rows[: count.value].copy(), tuple(dims) if has.value else None)
+
+    @staticmethod
+    def from_tables(albedo, emission=None, names=None):
+        """Rows for material ids 0..n from path-tracer tables of ids 1..n (id 0 is empty space)."""
+        albedo = np.asarray(albedo, np.float32).reshape(-1, 3)
+        rows = np.zeros(albedo.shape[0] + 1, MATERIAL_DTYPE)
+        rows[0]["name"] = b"EmptySpace"
+        rows[0]["flags"] = cabi.MATERIAL_HAS_NAME
+        rows["base_color"][0] = 0.8
+        rows["base_color"][1:] = albedo
+        rows["flags"][1:] = cabi.MATERIAL_HAS_BASE_COLOR
+        if emission is not None:
+            rows["emission"][1:] = np.asarray(emission, np.float32).reshape(-1, 3)
+            rows["flags"][1:] |= cabi.MATERIAL_HAS_EMISSION
+        for i, nm in enumerate(names or []):
+            rows[i + 1]["name"] = nm.encode()
+            rows[i + 1]["flags"] |= cabi.MATERIAL_HAS_NAME
+        return Materials(rows)
+
+    def save(self, path):
+        rows = np.ascontiguousarray(self.rows, MATERIAL_DTYPE)
+        check(lib().cqx_materials_save(os.fsencode(path), rows.ctypes.data_as(ctypes.c_void_p), len(rows)))
+
+    def colour_table(self):
+        """The viewer's 256 x 3 colour table (purple where no material is defined)."""
+        rows = np.ascontiguousarray(self.rows, MATERIAL_DTYPE)
+        out = np.zeros((256, 3), np.float32)
+        check(lib().cqx_materials_colour_table(rows.ctypes.data_as(ctypes.c_void_p), len(rows),
+                                               out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def pathtrace_tables(self):
+        """(albedo, emission) rows for material ids 1..n, the path tracer's tables."""
+        n = max(0, len(self.rows) - 1)
+        return (np.ascontiguousarray(self.rows["base_color"][1:], np.float32).reshape(n, 3),
+                np.ascontiguousarray(self.rows["emission"][1:], np.float32).reshape(n, 3))
 
 
 def material_table(n_materials, seed, n_emissive=1, emission=5.0):

@@ -60,6 +60,10 @@ class SphereEdit(ctypes.Structure):
     _fields_ = [("centre", ctypes.c_float * 3), ("radius", ctypes.c_float), ("material", ctypes.c_uint32)]
 
 
+MATERIAL_DTYPE = np.dtype([("name", "S64"), ("base_color", "<f4", (3,)), ("emission", "<f4", (3,)),
+                           ("flags", "<u4")])
+MATERIAL_HAS_NAME, MATERIAL_HAS_BASE_COLOR, MATERIAL_HAS_EMISSION = 1, 2, 4
+
 SPHERE_EDIT_DTYPE = np.dtype([("centre", "<f4", (3,)), ("radius", "<f4"), ("material", "<u4")])
 assert SPHERE_EDIT_DTYPE.itemsize == ctypes.sizeof(SphereEdit) == 20
 
@@ -100,6 +104,9 @@ SIGNATURES = {
     "cqx_hv_edit_spheres": (_ST, [_P, _P, _U64, _P]),
     "cqx_hv_compact": (_ST, [_P]),
     "cqx_hv_checkpoint": (_ST, [_P]),
+    "cqx_materials_load": (_ST, [ctypes.c_char_p, _P, _P, _P, _P]),
+    "cqx_materials_save": (_ST, [ctypes.c_char_p, _P, _U32]),
+    "cqx_materials_colour_table": (_ST, [_P, _U32, _P]),
     "cqx_hv_undo": (_ST, [_P]),
     "cqx_hv_redo": (_ST, [_P]),
     "cqx_volume_update": (_ST, [_P, _P, ctypes.POINTER(ctypes.c_uint32), _U64, _U64, _U32, _P]),

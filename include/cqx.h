@@ -34,6 +34,7 @@ typedef int32_t cqx_status;
 #define CQX_ERR_IO (-5)          /* file could not be read or written */
 #define CQX_ERR_NO_DEVICE (-6)   /* no usable GPU */
 #define CQX_ERR_LIMIT (-7)       /* input exceeds a documented limit */
+#define CQX_ERR_PARSE (-8)       /* a metadata file is not valid (see cqx_materials_load) */
 
 /* Number of material ids; node indices below this are material (leaf) nodes whose children
  * all point at themselves. Matches Internals::MaterialCount, src/library/storage.h:55. */
@@ -188,6 +189,34 @@ cqx_status cqx_hv_compact(cqx_host_volume* hv);
 cqx_status cqx_hv_checkpoint(cqx_host_volume* hv);
 cqx_status cqx_hv_undo(cqx_host_volume* hv);
 cqx_status cqx_hv_redo(cqx_host_volume* hv);
+
+/* ---- material metadata ------------------------------------------------------------------------
+ * The .toml file next to a .dag (Metadata::load / Metadata::save,
+ * src/application/base/metadata.cpp:34-109; Material, metadata.h:11-35): an optional
+ * `dimensions = [x, y, z]` and an array of [[materials]] tables, one per material id from 0, each
+ * with an optional `name` and `base_color` (missing colour: Material::DefaultColor 0.8 grey) and,
+ * as an extension for the path tracer, an optional `emission` (missing: 0). Other keys and tables
+ * are skipped. */
+#define CQX_MATERIAL_HAS_NAME 1u
+#define CQX_MATERIAL_HAS_BASE_COLOR 2u
+#define CQX_MATERIAL_HAS_EMISSION 4u
+typedef struct {
+    char name[64]; /* NUL-terminated */
+    float base_color[3];
+    float emission[3];
+    uint32_t flags; /* CQX_MATERIAL_HAS_* : which keys the file holds */
+} cqx_material;
+
+/* Reads up to 256 materials into `materials` (capacity 256) and their number into `count`.
+ * `dimensions` / `has_dimensions` (may be NULL) receive the dimensions key if present (a DAG's
+ * metadata should not hold one; serialize.cpp:64-69 ignores it). CQX_ERR_PARSE names the line. */
+cqx_status cqx_materials_load(const char* path, cqx_material* materials, uint32_t* count, int32_t dimensions[3],
+                              uint32_t* has_dimensions);
+/* Writes the materials in Metadata::save's layout (colours with four decimals). */
+cqx_status cqx_materials_save(const char* path, const cqx_material* materials, uint32_t count);
+/* The viewer's 256-entry colour table (Viewer::onVolumeModified, viewer.cpp:49-54): material i's
+ * base_color for i < count, purple (1, 0, 1) for the rest. */
+cqx_status cqx_materials_colour_table(const cqx_material* materials, uint32_t count, float colours[256 * 3]);
 
 /* Build the canonical (baked) DAG of a dense grid of material ids, x fastest then y then z,
  * whose voxel (0,0,0) sits at signed voxel coordinates `lower` (any placement, including across
