@@ -112,12 +112,21 @@ struct BatchSource {
 __device__ __forceinline__ const TraceParams& uniform_params(const SingleSource& s, const TraceParams&) { return s.P; }
 __device__ __forceinline__ const TraceParams& uniform_params(const BatchSource&, const TraceParams& lane) { return lane; }
 
-template <typename Source, bool kCache>
+template <typename Source, bool kCache, bool kTop = false>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
-                                                              uint32_t regroup, uint32_t refill, uint32_t rows)
+                                                              uint32_t regroup, uint32_t refill, uint32_t rows,
+                                                              uint32_t top_nodes)
 {
     extern __shared__ uint32_t lds_stack[];
+    // kTop: after the stacks, a copy of internal nodes [256, 256 + top_nodes) (the top levels of
+    // a breadth-first store, read by most rays) so that those child reads stay out of L2.
+    uint32_t* const top = lds_stack + rows * blockDim.x;
+    if constexpr (kTop) { // single-volume launches only
+        const uint32_t* g = src.P.nodes + kMaterialCount * 8;
+        for (uint32_t i = threadIdx.x; i < top_nodes * 8; i += blockDim.x) top[i] = g[i];
+        __syncthreads();
+    }
     if (n_dev) {
         // ray count produced on the device by an earlier kernel (path-tracer bounce loop)
         const uint64_t nd = *n_dev;
@@ -179,7 +188,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
         }
         const TraceParams& PP = uniform_params(src, P);
         for (uint32_t s = 0; s < regroup; s++)
-            if (S.phase == kWalk) walk_step<kCache>(PP, S, stack, stride);
+            if (S.phase == kWalk) walk_step<kCache, kTop>(PP, S, stack, stride, top, top_nodes);
     }
 }
 
@@ -233,20 +242,32 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         const uint64_t blocks = (n + block - 1) / block;
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
         hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
-    } else if (variant == 2 || variant == 3) {
+    } else if (variant >= 2 && variant <= 4) {
         uint32_t blocks, regroup, refill;
         cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill);
         if (st != CQX_OK) return st;
+        uint32_t top_nodes = 0;
+        size_t lds_total = lds;
+        if (variant == 4) {
+            const uint64_t internal = vol->node_count - CQX_MATERIAL_COUNT;
+            top_nodes = uint32_t(std::min<uint64_t>((cfg && cfg->lds_nodes) ? cfg->lds_nodes : 512u, internal));
+            lds_total += size_t(top_nodes) * 32;
+            if (lds_total > size_t(ctx->lds_per_block))
+                return fail(CQX_ERR_LIMIT, "stack and %u LDS nodes need %zu B of LDS per block", top_nodes, lds_total);
+        }
         SingleSource src{ P, rays, hits };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter));
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
         if (variant == 2)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill, rows);
-        else
+                               n, n_dev, counters, regroup, refill, rows, 0u);
+        else if (variant == 3)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, true>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill, rows);
+                               n, n_dev, counters, regroup, refill, rows, 0u);
+        else
+            hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false, true>), dim3(blocks), dim3(block), lds_total,
+                               stream, src, n, n_dev, counters, regroup, refill, rows, top_nodes);
     } else {
         return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
     }
@@ -509,10 +530,10 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
     if (cfg && cfg->variant == 3)
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill, rows);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u);
     else
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, false>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill, rows);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u);
     HIP_TRY(hipGetLastError());
     return CQX_OK;
 }

@@ -223,9 +223,11 @@ __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, 
     octant_advance(S);
 }
 
-// One iteration of the ESVO loop, raytracing.cpp:253-367.
-template <bool kCache = false>
-__device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+// One iteration of the ESVO loop, raytracing.cpp:253-367. With kTop, the children of internal
+// nodes [256, 256 + top_nodes) are read from `top`, their copy in LDS.
+template <bool kCache = false, bool kTop = false>
+__device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                          const uint32_t* top = nullptr, uint32_t top_nodes = 0)
 {
     if (S.iters >= P.max_iters) {
         S.phase = kAnomaly;
@@ -238,7 +240,10 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
     const float t1z = (float(wadd(S.cpz, csize)) - S.oz) * S.iz;
     const float tcx = min3f(t1x, t1y, t1z);
     const uint32_t slot = S.cid ^ S.sign_bits;
-    const uint32_t child = kCache ? pick8(S.nc, slot) : P.nodes[uint64_t(S.node) * 8 + slot];
+    uint32_t child;
+    if (kCache) child = pick8(S.nc, slot);
+    else if (kTop && S.node - kMaterialCount < top_nodes) child = top[(S.node - kMaterialCount) * 8 + slot];
+    else child = P.nodes[uint64_t(S.node) * 8 + slot];
     if (child > 0) {
         const bool internal = child >= kMaterialCount;
         if (internal && footprint_large(P, csize, tcx)) {

@@ -268,6 +268,8 @@ typedef struct {
     uint32_t regroup_interval; /* persistent: traversal steps between refill checks */
     uint32_t refill_lanes;     /* persistent: refill once at least this many lanes are idle */
     uint32_t max_iters;        /* per-ray iteration cap (0 = built-in 1<<22) */
+    uint32_t lds_nodes;        /* variant 4: internal nodes [256, 256 + lds_nodes) read from a copy in
+                                  LDS (the top of a breadth-first store; 0 = 512) */
 } cqx_launch_cfg;
 
 /* intersectVolume over n rays, batched: rays and hits are DEVICE pointers; stream is a

@@ -6,7 +6,9 @@
 //               walking lanes only; every `regroup` steps, once `refill` lanes wait (or none
 //               walks), waiting lanes move on to their next SubDAG, finished lanes emit their
 //               record, idle lanes take new rays (k_intersect_persistent)
-// Usage: core_emulation lane|persistent nodes.bin sub.bin rays.bin out.bin surf max_footprint
+//   top         persistent, with the children of internal nodes [256, 256 + 512) read from a copy
+//               (the LDS node cache of variant 4)
+// Usage: core_emulation lane|persistent|top nodes.bin sub.bin rays.bin out.bin surf max_footprint
 //                       max_iters refill regroup
 #define __device__
 #define __host__
@@ -40,7 +42,8 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "usage: %s lane|persistent nodes sub rays out surf fp max_iters refill regroup\n", argv[0]);
         return 2;
     }
-    const bool persistent = std::string_view(argv[1]) == "persistent";
+    const bool use_top = std::string_view(argv[1]) == "top";
+    const bool persistent = use_top || std::string_view(argv[1]) == "persistent";
     const std::vector<char> nb = slurp(argv[2]), sb = slurp(argv[3]), rb = slurp(argv[4]);
     if (nb.empty() || sb.size() != 8 * sizeof(cqx_subdag)) return 3;
     TraceParams P;
@@ -55,6 +58,9 @@ int main(int argc, char** argv)
     const uint64_t n = rb.size() / 24;
     std::vector<cqx_hit> out(n);
     std::vector<uint32_t> stack(64 * 33);
+    const uint64_t internal = nb.size() / 32 - kMaterialCount;
+    const uint32_t top_nodes = uint32_t(internal < 512 ? internal : 512);
+    const std::vector<uint32_t> top(P.nodes + kMaterialCount * 8, P.nodes + (kMaterialCount + top_nodes) * 8);
 
     if (!persistent) {
         for (uint64_t i = 0; i < n; i++) {
@@ -89,7 +95,10 @@ int main(int argc, char** argv)
             }
             for (int s = 0; s < regroup; s++)
                 for (int l = 0; l < 64; l++)
-                    if (S[l].phase == kWalk) walk_step(P, S[l], &stack[l], 64);
+                    if (S[l].phase == kWalk) {
+                        if (use_top) walk_step<false, true>(P, S[l], &stack[l], 64, top.data(), top_nodes);
+                        else walk_step(P, S[l], &stack[l], 64);
+                    }
         }
     }
     FILE* f = std::fopen(argv[5], "wb");

@@ -1,7 +1,8 @@
 """The device traversal core (cubiquity_amd/csrc/traverse_core.h) compiled for the host and driven
 the way the kernels drive it (tests/cpp/core_emulation.cpp): per-ray trace_ray, and 64-lane waves
 with the persistent kernel's service-pass policy (deferred SubDAG changes and records, refills
-between traversal steps) at several refill / regroup settings. Every 32-byte record must be
+between traversal steps) at several refill / regroup settings, also with the top nodes read from
+a copy (the LDS node cache). Every 32-byte record must be
 bit-identical to the oracle's, so a logic change to the core is caught on the CPU."""
 import os
 import subprocess
@@ -36,7 +37,7 @@ def _run(emu, tmp_path, cq, hv, rays, mode, surf, fp, max_iters, refill=32, regr
 
 
 @pytest.mark.parametrize("mode,refill,regroup", [("lane", 0, 0), ("persistent", 32, 8), ("persistent", 8, 1),
-                                                 ("persistent", 64, 3)])
+                                                 ("persistent", 64, 3), ("top", 16, 4)])
 def test_core_matches_oracle_128(emu, tmp_path, cq, oracle, vol128, rays128, mode, refill, regroup):
     sub = oracle.find_subdags(vol128.nodes, vol128.root)
     rays = np.concatenate([rays128["camera"][::5], rays128["incoherent"], rays128["special"]])
@@ -65,7 +66,7 @@ def test_core_small_and_degenerate_volumes(emu, tmp_path, cq, oracle):
         sp[:, :3] += np.asarray(lower, np.float32)
         rays = np.concatenate([_brute_rays(n, lower, 3000, 17), sp]).astype(np.float32)
         sub = oracle.find_subdags(hv.nodes, hv.root)
-        for mode in ("lane", "persistent"):
+        for mode in ("lane", "persistent", "top"):
             got = _run(emu, tmp_path, cq, hv, rays, mode, True, -1.0, 1 << 16, 16, 4)
             ref = oracle.intersect(hv.nodes, sub, rays, True, -1.0, max_iters=1 << 16)
             assert records_equal(got, ref).all(), f"{name} {mode}"
