@@ -31,7 +31,7 @@ struct cqx_ctx {
     int device = 0;
     int cu_count = 0;
     int lds_per_block = 0;
-    uint64_t* d_counter = nullptr; // work counters of the persistent kernel (8 queues)
+    uint64_t* d_counter = nullptr; // work counters of the persistent kernel (8 queues, 128 B apart)
     void* scratch = nullptr;       // staging for cqx_intersect_host
     size_t scratch_bytes = 0;
     hipStream_t capture_stream = nullptr; // private stream for graph capture

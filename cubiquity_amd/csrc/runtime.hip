@@ -50,10 +50,11 @@ __global__ void __launch_bounds__(256) k_intersect_lane(TraceParams P, const cqx
 // draw from their own counter; a wave whose queue is drained moves on to the next queue. A wave
 // exits when every queue is drained and all its lanes are idle.
 constexpr int kQueues = 8;
+constexpr int kCounterStride = 16; // queue counters 128 bytes apart: one cache line each
 
 __global__ void k_reset_counters(unsigned long long* counters)
 {
-    if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
+    if (threadIdx.x < kQueues) counters[threadIdx.x * kCounterStride] = 0;
 }
 
 // Where a persistent wave's ray ids come from. SingleSource: one volume, one ray array; its
@@ -116,7 +117,7 @@ template <typename Source, bool kCache, bool kTop = false>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
                                                               uint32_t regroup, uint32_t refill, uint32_t rows,
-                                                              uint32_t top_nodes)
+                                                              uint32_t top_nodes, uint32_t chunk)
 {
     extern __shared__ uint32_t lds_stack[];
     // kTop: after the stacks, a copy of internal nodes [256, 256 + top_nodes) (the top levels of
@@ -142,10 +143,12 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
     const uint64_t per_q = (n + kQueues - 1) / kQueues;
     uint32_t qc = blockIdx.x % kQueues; // current queue
     uint32_t drained = 0;               // queues found empty so far
+    // Ray ids [c_next, c_end) claimed by this wave: one atomic claims `chunk` ids, and service
+    // passes hand them to idle lanes without further atomics.
+    uint64_t c_next = 0, c_end = 0;
 
     RayState S;
     S.phase = kIdle;
-    bool exhausted = false;
 
     for (;;) {
         // Lanes are busy while they walk a SubDAG. Between SubDAGs, finished or idle, they wait
@@ -160,31 +163,37 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                 *dst = out;
                 S.phase = kIdle;
             }
-            if (!exhausted) {
-                const uint64_t idle = __ballot(S.phase == kIdle);
-                const uint32_t nidle = uint32_t(__popcll(idle));
+            // Claim a new chunk once the current one is used up (at most one claim per queue
+            // finds it empty, so this ends).
+            while (c_next == c_end && drained < kQueues) {
                 const uint64_t q_begin = per_q * qc;
                 const uint64_t q_end = q_begin + per_q < n ? q_begin + per_q : n;
                 const uint64_t q_len = q_end > q_begin ? q_end - q_begin : 0;
                 unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(&counters[qc], (unsigned long long)nidle);
+                if (lane == 0) base = atomicAdd(&counters[qc * kCounterStride], (unsigned long long)chunk);
                 base = __shfl(base, 0);
-                if (base + nidle >= q_len) {
+                if (base < q_len) {
+                    c_next = q_begin + base;
+                    c_end = q_begin + (base + chunk < q_len ? base + chunk : q_len);
+                } else {
                     qc = (qc + 1) % kQueues;
-                    if (++drained == kQueues) exhausted = true;
-                }
-                if (S.phase == kIdle) {
-                    const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
-                    const uint64_t k = base + rank;
-                    if (k < q_len) {
-                        const float* r;
-                        src.fetch(q_begin + k, P, r, dst);
-                        ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
-                        enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
-                    }
+                    drained++;
                 }
             }
-            if (exhausted && __ballot(S.phase != kIdle) == 0) break;
+            const uint64_t idle = __ballot(S.phase == kIdle);
+            if (S.phase == kIdle) {
+                const uint32_t rank = uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                const uint64_t k = c_next + rank;
+                if (k < c_end) {
+                    const float* r;
+                    src.fetch(k, P, r, dst);
+                    ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
+                    enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
+                }
+            }
+            const uint64_t nidle = uint64_t(__popcll(idle));
+            c_next = c_next + nidle < c_end ? c_next + nidle : c_end;
+            if (drained == kQueues && __ballot(S.phase != kIdle) == 0) break;
         }
         const TraceParams& PP = uniform_params(src, P);
         for (uint32_t s = 0; s < regroup; s++)
@@ -193,8 +202,9 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
 }
 
 // Launch shape of the persistent kernel: CUs x waves_per_cu waves, at most one lane per ray.
+// `chunk`: ray ids a wave claims per atomic, about a quarter of a wave's share (64 to 4096).
 cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint32_t block, uint64_t n,
-                            uint32_t& blocks, uint32_t& regroup, uint32_t& refill)
+                            uint32_t& blocks, uint32_t& regroup, uint32_t& refill, uint32_t& chunk)
 {
     const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
     regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
@@ -204,6 +214,9 @@ cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint3
     const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
     const uint64_t need_blocks = (n + block - 1) / block;
     blocks = uint32_t(std::max<uint64_t>(1, std::min(want_blocks, need_blocks)));
+    const uint64_t waves = uint64_t(blocks) * (block / 64);
+    const uint64_t share = (n / (waves * 4) + 63) / 64 * 64;
+    chunk = uint32_t(std::min<uint64_t>(4096, std::max<uint64_t>(64, share)));
     return CQX_OK;
 }
 
@@ -243,8 +256,8 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
         hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
     } else if (variant >= 2 && variant <= 4) {
-        uint32_t blocks, regroup, refill;
-        cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill);
+        uint32_t blocks, regroup, refill, chunk;
+        cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill, chunk);
         if (st != CQX_OK) return st;
         uint32_t top_nodes = 0;
         size_t lds_total = lds;
@@ -261,13 +274,13 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
         if (variant == 2)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill, rows, 0u);
+                               n, n_dev, counters, regroup, refill, rows, 0u, chunk);
         else if (variant == 3)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, true>), dim3(blocks), dim3(block), lds, stream, src,
-                               n, n_dev, counters, regroup, refill, rows, 0u);
+                               n, n_dev, counters, regroup, refill, rows, 0u, chunk);
         else
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false, true>), dim3(blocks), dim3(block), lds_total,
-                               stream, src, n, n_dev, counters, regroup, refill, rows, top_nodes);
+                               stream, src, n, n_dev, counters, regroup, refill, rows, top_nodes, chunk);
     } else {
         return fail(CQX_ERR_INVALID_ARG, "unknown kernel variant %u", variant);
     }
@@ -294,7 +307,7 @@ cqx_status cqx_create(int device, cqx_ctx** out)
     c->device = device;
     c->cu_count = prop.multiProcessorCount;
     c->lds_per_block = int(prop.sharedMemPerBlock);
-    if (hipMalloc(&c->d_counter, sizeof(uint64_t) * 64) != hipSuccess) {
+    if (hipMalloc(&c->d_counter, sizeof(uint64_t) * kQueues * kCounterStride) != hipSuccess) {
         delete c;
         return fail(CQX_ERR_OOM, "device allocation of the work counters failed");
     }
@@ -521,8 +534,8 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
         ctx->batch_bytes = bytes;
     }
     HIP_TRY(hipMemcpy(ctx->batch_desc, desc.data(), bytes, hipMemcpyHostToDevice));
-    uint32_t blocks, regroup, refill;
-    st = persistent_shape(ctx, cfg, block, total, blocks, regroup, refill);
+    uint32_t blocks, regroup, refill, chunk;
+    st = persistent_shape(ctx, cfg, block, total, blocks, regroup, refill, chunk);
     if (st != CQX_OK) return st;
     BatchSource src{ static_cast<const BatchDesc*>(ctx->batch_desc), n_batches,
                      (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters };
@@ -530,10 +543,10 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
     if (cfg && cfg->variant == 3)
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u, chunk);
     else
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, false>), dim3(blocks), dim3(block), lds, s, src, total,
-                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u);
+                           (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u, chunk);
     HIP_TRY(hipGetLastError());
     return CQX_OK;
 }
