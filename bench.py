@@ -154,6 +154,43 @@ def bench_batched(cq, W, ctx, reps):
             "separate_launches_ms": round(sep, 3), "mrays_per_s": round(total / (batched * 1e-3) / 1e6, 1)}
 
 
+def bench_edit_frames(cq, W, ctx, frames):
+    """Edit frames on the 2048^3 volume: per frame 1000 sphere edits on the host
+    (cqx_hv_edit_spheres, copy-on-write + hash-consing), the append-only device update
+    (cqx_volume_update) and 2^24 incoherent rays traced against the edited volume."""
+    import torch
+
+    w = W.EDIT_2048
+    t0 = time.time()
+    hv = cq.HostVolume.generate(w.params())
+    build_s = time.time() - t0
+    rays = torch.from_numpy(cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)).cuda()
+    hits = torch.empty((rays.shape[0], 32), dtype=torch.uint8, device="cuda")
+    dv = ctx.upload(hv)
+    nodes0 = hv.node_count
+    trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
+    before = timed(trace, 3)
+    edit_ms, update_ms, trace_ms, appended = [], [], [], []
+    for f in range(frames):
+        e = W.edit_frame_spheres(f)
+        t = time.perf_counter()
+        first = hv.edit_spheres(e)
+        edit_ms.append((time.perf_counter() - t) * 1e3)
+        t = time.perf_counter()
+        dv.update(hv, first)
+        update_ms.append((time.perf_counter() - t) * 1e3)
+        appended.append(hv.node_count - first)
+        trace_ms.append(timed(trace, 3))
+    dv.free()
+    mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
+    return {"volume": "2048^3 seed 21, 16 materials", "edits_per_frame": W.EDIT_FRAME_EDITS, "frames": frames,
+            "build_s": round(build_s, 2), "nodes_before": nodes0, "nodes_after": hv.node_count,
+            "host_edit_ms": mean(edit_ms), "edits_per_s": round(W.EDIT_FRAME_EDITS / (np.mean(edit_ms) * 1e-3)),
+            "appended_nodes_per_frame": int(np.mean(appended)), "device_update_ms": mean(update_ms),
+            "trace_ms_before_edits": round(before, 3), "trace_ms": mean(trace_ms),
+            "frame_ms": round(float(np.mean(edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 3)}
+
+
 def bench_adjoint(cq, W, ctx, reps):
     """Differentiable rendering (512^3, 16 materials, 512x512, 8 spp, 3 bounces): forward time,
     adjoint (path replay) time and their ratio."""
@@ -198,6 +235,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-edits", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch defaults)")
     ap.add_argument("--launch", default="", help="explicit launch, e.g. variant=2,waves_per_cu=20,regroup_interval=8")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -326,6 +364,9 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(cq, W, ctx, max(2, args.steps // 4))
         log(f"batched: {out['batched']}")
+    if not args.no_edits:
+        out["edit_frames_2048"] = bench_edit_frames(cq, W, ctx, 3)
+        log(f"edit frames: {out['edit_frames_2048']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hv, rays, hits, cq)
         out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)

@@ -56,6 +56,28 @@ DIFFRENDER_512 = Workload("diffrender_512", 9, 5, {"n_materials": 16}, _camera(9
 LARGE_16384 = Workload("large_16384", 14, 9, {"n_materials": 255, "n_spheres": 1 << 20}, None, 7, 1 << 26)
 
 
+# Edit frames (a scaled form of the 16384^3 / 1M-sphere / 10k-edits-per-frame workload, whose
+# volume does not fit a 64 GB build host): a 2048^3 volume, then per frame 1000 sphere edits
+# (seeded by frame) applied on the host and uploaded as an append-only delta, then 2^24
+# incoherent rays traced against the edited volume.
+EDIT_2048 = Workload("edit_2048", 11, 21, {"n_materials": 16}, None, 23, 1 << 24)
+EDIT_FRAME_EDITS = 1000
+
+
+def edit_frame_spheres(frame, log2_size=11, n_materials=16, n=EDIT_FRAME_EDITS):
+    """(n, 5) edits of one frame: centre U[0, N)^3, radius U[2, 16] * (N / 2048)^(1/3), material
+    U{0..n_materials} (0 carves), numpy PCG64(7000 + frame)."""
+    import numpy as np
+
+    rng = np.random.Generator(np.random.PCG64(7000 + frame))
+    side = float(1 << log2_size)
+    e = np.zeros((n, 5))
+    e[:, :3] = rng.uniform(0.0, side, (n, 3))
+    e[:, 3] = rng.uniform(2.0, 16.0, n) * (side / 2048.0) ** (1.0 / 3.0)
+    e[:, 4] = rng.integers(0, n_materials + 1, n)
+    return e
+
+
 def batched_volumes():
     """The 64 volumes of the batched workload: seed 100+i, side 2^(6 + i mod 6)."""
     return [Workload(f"batched_{i}", 6 + i % 6, 100 + i, {}, _camera(6 + i % 6, 256, 256), 1000 + i, 0)
@@ -70,4 +92,4 @@ def batched_sizes(seed=42):
     return [int(round(10 ** u)) for u in rng.uniform(3.0, 6.0, 64)]
 
 
-ALL = {w.name: w for w in (RAYCAST_128, PATHTRACE_1024, DIFFRENDER_512, LARGE_16384)}
+ALL = {w.name: w for w in (RAYCAST_128, PATHTRACE_1024, DIFFRENDER_512, LARGE_16384, EDIT_2048)}
