@@ -92,15 +92,15 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
     cqx_launch_cfg cur{};
     cur.variant = 2;
     cur.block = 256;
-    cur.waves_per_cu = 16;
-    cur.regroup_interval = 16;
-    cur.refill_lanes = 32;
+    cur.waves_per_cu = 32;
+    cur.regroup_interval = 8;
+    cur.refill_lanes = 16;
     float cur_ms = 0;
     cqx_status st = time_cfg(ctx, vol, rays, n, flags, max_footprint, hits, cur, s, cur_ms);
     if (st != CQX_OK) return st;
 
     const uint32_t blocks[] = { 64, 128, 256 };
-    const uint32_t waves[] = { 4, 8, 12, 16, 20, 24, 32 };
+    const uint32_t waves[] = { 4, 8, 12, 16, 20, 24, 28, 32 };
     const uint32_t regroups[] = { 1, 4, 8, 16, 32, 64 };
     const uint32_t refills[] = { 8, 16, 32, 48, 64 };
     auto sweep = [&](uint32_t cqx_launch_cfg::*field, const uint32_t* vals, size_t count) -> cqx_status {
@@ -122,7 +122,7 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
     const uint32_t variants[] = { 2, 3 };
     for (int pass = 0; pass < 2; pass++) {
         if ((st = sweep(&cqx_launch_cfg::variant, variants, 2)) != CQX_OK) return st;
-        if ((st = sweep(&cqx_launch_cfg::waves_per_cu, waves, 7)) != CQX_OK) return st;
+        if ((st = sweep(&cqx_launch_cfg::waves_per_cu, waves, 8)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::regroup_interval, regroups, 6)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::refill_lanes, refills, 5)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::block, blocks, 3)) != CQX_OK) return st;

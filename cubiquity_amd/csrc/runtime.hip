@@ -206,9 +206,10 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
 cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint32_t block, uint64_t n,
                             uint32_t& blocks, uint32_t& regroup, uint32_t& refill, uint32_t& chunk)
 {
-    const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
-    regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 16u;
-    refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 32u;
+    // built-in defaults: the best launch of a sweep on the 1024^3 incoherent workload
+    const uint32_t waves_per_cu = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 32u;
+    regroup = (cfg && cfg->regroup_interval) ? cfg->regroup_interval : 8u;
+    refill = (cfg && cfg->refill_lanes) ? cfg->refill_lanes : 16u;
     if (waves_per_cu > 64 || regroup > 4096 || refill > 64)
         return fail(CQX_ERR_INVALID_ARG, "persistent launch parameters out of range");
     const uint64_t want_blocks = (uint64_t(ctx->cu_count) * waves_per_cu * 64 + block - 1) / block;
