@@ -119,9 +119,9 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
         }
         return CQX_OK;
     };
-    const uint32_t variants[] = { 2, 3 };
+    const uint32_t variants[] = { 2, 3, 5 };
     for (int pass = 0; pass < 2; pass++) {
-        if ((st = sweep(&cqx_launch_cfg::variant, variants, 2)) != CQX_OK) return st;
+        if ((st = sweep(&cqx_launch_cfg::variant, variants, 3)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::waves_per_cu, waves, 8)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::regroup_interval, regroups, 6)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::refill_lanes, refills, 5)) != CQX_OK) return st;

@@ -201,6 +201,98 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
     }
 }
 
+// Variant 5: the persistent kernel with two rays per lane (slots 0 and 1, each with its own
+// state and stack column). Each traversal step issues the child reads of both slots before it
+// waits for either, so a wave keeps twice as many dependent loads in flight; service passes and
+// chunked claims are those of k_intersect_persistent, counted over both slots.
+__global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64_t n, const uint32_t* __restrict__ n_dev,
+                                                        unsigned long long* __restrict__ counters, uint32_t regroup,
+                                                        uint32_t refill, uint32_t rows, uint32_t chunk)
+{
+    extern __shared__ uint32_t lds_stack[];
+    if (n_dev) {
+        const uint64_t nd = *n_dev;
+        n = nd < n ? nd : n;
+    }
+    const TraceParams& P = src.P;
+    const uint32_t lane = threadIdx.x & 63u;
+    // Each wave owns 2 x rows x 64 words: slot j's column starts at (j * rows) * 64 + lane.
+    uint32_t* const wave_stack = lds_stack + (threadIdx.x >> 6) * 2u * rows * 64u + lane;
+    uint32_t* const stk[2] = { wave_stack, wave_stack + rows * 64u };
+    constexpr uint32_t stride = 64u;
+    const uint64_t per_q = (n + kQueues - 1) / kQueues;
+    uint32_t qc = blockIdx.x % kQueues;
+    uint32_t drained = 0;
+    uint64_t c_next = 0, c_end = 0;
+
+    RayState S[2];
+    cqx_hit* dst[2] = { nullptr, nullptr };
+    S[0].phase = kIdle;
+    S[1].phase = kIdle;
+    TraceParams unused;
+
+    for (;;) {
+        const uint32_t walking = uint32_t(__popcll(__ballot(S[0].phase == kWalk))) +
+                                 uint32_t(__popcll(__ballot(S[1].phase == kWalk)));
+        if (128u - walking >= refill || walking == 0) {
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                enter_octants(P, S[j], stk[j], stride);
+                if (is_finished(S[j].phase)) {
+                    cqx_hit out;
+                    emit_record(P, S[j], out);
+                    *dst[j] = out;
+                    S[j].phase = kIdle;
+                }
+            }
+            while (c_next == c_end && drained < kQueues) {
+                const uint64_t q_begin = per_q * qc;
+                const uint64_t q_end = q_begin + per_q < n ? q_begin + per_q : n;
+                const uint64_t q_len = q_end > q_begin ? q_end - q_begin : 0;
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(&counters[qc * kCounterStride], (unsigned long long)chunk);
+                base = __shfl(base, 0);
+                if (base < q_len) {
+                    c_next = q_begin + base;
+                    c_end = q_begin + (base + chunk < q_len ? base + chunk : q_len);
+                } else {
+                    qc = (qc + 1) % kQueues;
+                    drained++;
+                }
+            }
+            const uint64_t idle0 = __ballot(S[0].phase == kIdle);
+            const uint64_t idle1 = __ballot(S[1].phase == kIdle);
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t n0 = uint32_t(__popcll(idle0));
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                if (S[j].phase == kIdle) {
+                    const uint32_t rank = j == 0 ? uint32_t(__popcll(idle0 & below))
+                                                 : n0 + uint32_t(__popcll(idle1 & below));
+                    const uint64_t k = c_next + rank;
+                    if (k < c_end) {
+                        const float* r;
+                        src.fetch(k, unused, r, dst[j]);
+                        ray_setup(S[j], r[0], r[1], r[2], r[3], r[4], r[5]);
+                        enter_octants(P, S[j], stk[j], stride);
+                    }
+                }
+            }
+            const uint64_t nidle = uint64_t(n0) + uint64_t(__popcll(idle1));
+            c_next = c_next + nidle < c_end ? c_next + nidle : c_end;
+            if (drained == kQueues && __ballot(S[0].phase != kIdle || S[1].phase != kIdle) == 0) break;
+        }
+        for (uint32_t s = 0; s < regroup; s++) {
+            const bool w0 = S[0].phase == kWalk, w1 = S[1].phase == kWalk;
+            uint32_t c0 = 0, c1 = 0;
+            if (w0) c0 = walk_fetch(P, S[0]);
+            if (w1) c1 = walk_fetch(P, S[1]);
+            if (w0) walk_step_on(P, S[0], stk[0], stride, c0);
+            if (w1) walk_step_on(P, S[1], stk[1], stride, c1);
+        }
+    }
+}
+
 // Launch shape of the persistent kernel: CUs x waves_per_cu waves, at most one lane per ray.
 // `chunk`: ray ids a wave claims per atomic, about a quarter of a wave's share (64 to 4096).
 cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint32_t block, uint64_t n,
@@ -256,6 +348,18 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         const uint64_t blocks = (n + block - 1) / block;
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
         hipLaunchKernelGGL(k_intersect_lane, dim3(uint32_t(blocks)), dim3(block), lds, stream, P, rays, n, n_dev, hits);
+    } else if (variant == 5) {
+        uint32_t blocks, regroup, refill, chunk;
+        cqx_status st = persistent_shape(ctx, cfg, block, (n + 1) / 2, blocks, regroup, refill, chunk);
+        if (st != CQX_OK) return st;
+        const size_t lds2 = 2 * lds;
+        if (lds2 > size_t(ctx->lds_per_block))
+            return fail(CQX_ERR_LIMIT, "two stacks per lane need %zu B of LDS per block", lds2);
+        SingleSource src{ P, rays, hits };
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
+        hipLaunchKernelGGL(k_intersect_pair, dim3(blocks), dim3(block), lds2, stream, src, n, n_dev,
+                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill, rows, chunk);
     } else if (variant >= 2 && variant <= 4) {
         uint32_t blocks, regroup, refill, chunk;
         cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill, chunk);

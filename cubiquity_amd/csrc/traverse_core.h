@@ -41,21 +41,19 @@ __device__ __forceinline__ int find_msb(uint32_t v) { return v ? 31 - __builtin_
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
 __device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return int32_t(uint32_t(a) - uint32_t(b)); }
 __device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
-__device__ __forceinline__ int32_t floor_to(int32_t x, int h)
-{
-    h &= 31;
-    return int32_t(uint32_t(x >> h) << h);
-}
 
-// findFirstChild, raytracing.cpp:178-196; child id bits x=1, y=2, z=4.
+// findFirstChild, raytracing.cpp:178-196; child id bits x=1, y=2, z=4, also returned as flags.
+// The `entry <= 0` term (children behind the origin) is combined as a lane mask, not a branch.
 __device__ __forceinline__ uint32_t first_child(float entry, float ox, float oy, float oz, float ix, float iy,
-                                                float iz, int32_t cx, int32_t cy, int32_t cz)
+                                                float iz, int32_t cx, int32_t cy, int32_t cz, bool& bx, bool& by,
+                                                bool& bz)
 {
     const float fx = float(cx), fy = float(cy), fz = float(cz);
-    uint32_t id = ((fx - ox) * ix < entry ? 1u : 0u) | ((fy - oy) * iy < entry ? 2u : 0u) |
-                  ((fz - oz) * iz < entry ? 4u : 0u);
-    if (entry <= 0.0f) id |= (ox >= fx ? 1u : 0u) | (oy >= fy ? 2u : 0u) | (oz >= fz ? 4u : 0u);
-    return id;
+    const bool behind = entry <= 0.0f;
+    bx = ((fx - ox) * ix < entry) | (behind & (ox >= fx));
+    by = ((fy - oy) * iy < entry) | (behind & (oy >= fy));
+    bz = ((fz - oz) * iz < entry) | (behind & (oz >= fz));
+    return (bx ? 1u : 0u) | (by ? 2u : 0u) | (bz ? 4u : 0u);
 }
 
 // First voxel of the hit block entered by the ray (reflected shifted coordinates).
@@ -88,8 +86,8 @@ __host__ __device__ inline bool footprint_is_disabled(float max_footprint) { ret
 // RN(c / t) > -1 holds exactly when t > 0, t == +0 or t < -c, which needs no division.
 __device__ __forceinline__ bool footprint_large(const TraceParams& P, int32_t csize, float tcx)
 {
-    if (P.fp_disabled && csize > 0)
-        return (tcx > 0.0f) || (tcx == 0.0f && (__float_as_uint(tcx) >> 31) == 0u) || (tcx < -float(csize));
+    if (P.fp_disabled && csize > 0) // t > 0 or t == +0: sign bit clear and not NaN
+        return __float_as_uint(tcx) <= 0x7f800000u || tcx < -float(csize);
     return (float(csize) / tcx) > P.max_footprint;
 }
 
@@ -208,12 +206,12 @@ __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, 
             S.node = sd.node;
             if (kCache) load_node(P, S.node, S.nc);
             S.csize = int32_t(node_size / 2);
-            const uint32_t cid = first_child(entry, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(px, S.csize),
-                                             wadd(py, S.csize), wadd(pz, S.csize));
-            S.cid = cid;
-            S.cpx = wadd(px, (cid & 1u) ? S.csize : 0);
-            S.cpy = wadd(py, (cid & 2u) ? S.csize : 0);
-            S.cpz = wadd(pz, (cid & 4u) ? S.csize : 0);
+            bool bx, by, bz;
+            S.cid = first_child(entry, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(px, S.csize), wadd(py, S.csize),
+                                wadd(pz, S.csize), bx, by, bz);
+            S.cpx = wadd(px, bx ? S.csize : 0);
+            S.cpy = wadd(py, by ? S.csize : 0);
+            S.cpz = wadd(pz, bz ? S.csize : 0);
             S.last_exit = exitt;
             S.entries++;
             S.phase = kWalk;
@@ -223,11 +221,24 @@ __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, 
     octant_advance(S);
 }
 
-// One iteration of the ESVO loop, raytracing.cpp:253-367. With kTop, the children of internal
-// nodes [256, 256 + top_nodes) are read from `top`, their copy in LDS.
+// The child read of a walk step (raytracing.cpp:265): slot cid ^ signBits of the current node.
+// With kTop, the children of internal nodes [256, 256 + top_nodes) are read from `top`, their
+// copy in LDS. Split from walk_step_on so that a lane walking two rays can issue both reads
+// before it waits for either.
 template <bool kCache = false, bool kTop = false>
-__device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                          const uint32_t* top = nullptr, uint32_t top_nodes = 0)
+__device__ __forceinline__ uint32_t walk_fetch(const TraceParams& P, const RayState& S, const uint32_t* top = nullptr,
+                                               uint32_t top_nodes = 0)
+{
+    const uint32_t slot = S.cid ^ S.sign_bits;
+    if (kCache) return pick8(S.nc, slot);
+    if (kTop && S.node - kMaterialCount < top_nodes) return top[(S.node - kMaterialCount) * 8 + slot];
+    return P.nodes[uint64_t(S.node) * 8 + slot];
+}
+
+// One iteration of the ESVO loop, raytracing.cpp:253-367, given the child that walk_fetch read.
+template <bool kCache = false>
+__device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                             uint32_t child)
 {
     if (S.iters >= P.max_iters) {
         S.phase = kAnomaly;
@@ -239,11 +250,6 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
     const float t1y = (float(wadd(S.cpy, csize)) - S.oy) * S.iy;
     const float t1z = (float(wadd(S.cpz, csize)) - S.oz) * S.iz;
     const float tcx = min3f(t1x, t1y, t1z);
-    const uint32_t slot = S.cid ^ S.sign_bits;
-    uint32_t child;
-    if (kCache) child = pick8(S.nc, slot);
-    else if (kTop && S.node - kMaterialCount < top_nodes) child = top[(S.node - kMaterialCount) * 8 + slot];
-    else child = P.nodes[uint64_t(S.node) * 8 + slot];
     if (child > 0) {
         const bool internal = child >= kMaterialCount;
         if (internal && footprint_large(P, csize, tcx)) {
@@ -262,12 +268,12 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
             const int32_t half = int32_t(uint32_t(csize) >> 1); // csize = 2^(height - 1) >= 2 here
             S.csize = half;
             S.entries++;
-            const uint32_t cid = first_child(tce, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(S.cpx, half),
-                                             wadd(S.cpy, half), wadd(S.cpz, half));
-            S.cid = cid;
-            S.cpx = wadd(S.cpx, (cid & 1u) ? half : 0);
-            S.cpy = wadd(S.cpy, (cid & 2u) ? half : 0);
-            S.cpz = wadd(S.cpz, (cid & 4u) ? half : 0);
+            bool bx, by, bz;
+            S.cid = first_child(tce, S.ox, S.oy, S.oz, S.ix, S.iy, S.iz, wadd(S.cpx, half), wadd(S.cpy, half),
+                                wadd(S.cpz, half), bx, by, bz);
+            S.cpx = wadd(S.cpx, bx ? half : 0);
+            S.cpy = wadd(S.cpy, by ? half : 0);
+            S.cpz = wadd(S.cpz, bz ? half : 0);
             return;
         }
         // Hit (raytracing.cpp:302-320): the record is built by emit_record.
@@ -300,12 +306,22 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
         const uint32_t bx = uint32_t(S.cpx >> msb) & 1u, by = uint32_t(S.cpy >> msb) & 1u,
                        bz = uint32_t(S.cpz >> msb) & 1u;
         S.cid = bx | (by << 1) | (bz << 2);
-        S.cpx = wadd(floor_to(S.cpx, height), bx ? ns : 0);
-        S.cpy = wadd(floor_to(S.cpy, height), by ? ns : 0);
-        S.cpz = wadd(floor_to(S.cpz, height), bz ? ns : 0);
+        // (p >> height << height) + bit msb of p * 2^msb is p with the bits below msb cleared
+        // (msb <= 30 here: a crossing at bit 31 is above every SubDAG and left the walk above).
+        S.cpx &= -ns;
+        S.cpy &= -ns;
+        S.cpz &= -ns;
         S.last_exit = 0.0f;
         S.entries++;
     }
+}
+
+// One iteration of the ESVO loop (read and step).
+template <bool kCache = false, bool kTop = false>
+__device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                          const uint32_t* top = nullptr, uint32_t top_nodes = 0)
+{
+    walk_step_on<kCache>(P, S, stack, stride, walk_fetch<kCache, kTop>(P, S, top, top_nodes));
 }
 
 // Moves a lane that is between SubDAGs (phase kLeave or kOctant) into the walk of the next SubDAG

@@ -8,6 +8,8 @@
 //               record, idle lanes take new rays (k_intersect_persistent)
 //   top         persistent, with the children of internal nodes [256, 256 + 512) read from a copy
 //               (the LDS node cache of variant 4)
+//   pair        persistent with two rays per lane (variant 5): both slots' child reads are taken
+//               before either slot steps on its read
 // Usage: core_emulation lane|persistent|top nodes.bin sub.bin rays.bin out.bin surf max_footprint
 //                       max_iters refill regroup
 #define __device__
@@ -43,6 +45,7 @@ int main(int argc, char** argv)
         return 2;
     }
     const bool use_top = std::string_view(argv[1]) == "top";
+    const bool pair = std::string_view(argv[1]) == "pair";
     const bool persistent = use_top || std::string_view(argv[1]) == "persistent";
     const std::vector<char> nb = slurp(argv[2]), sb = slurp(argv[3]), rb = slurp(argv[4]);
     if (nb.empty() || sb.size() != 8 * sizeof(cqx_subdag)) return 3;
@@ -62,7 +65,51 @@ int main(int argc, char** argv)
     const uint32_t top_nodes = uint32_t(internal < 512 ? internal : 512);
     const std::vector<uint32_t> top(P.nodes + kMaterialCount * 8, P.nodes + (kMaterialCount + top_nodes) * 8);
 
-    if (!persistent) {
+    if (pair) {
+        // 64 lanes x 2 slots; slot j of lane l uses stack column 2 * l + j
+        std::vector<uint32_t> stack2(128 * 33);
+        RayState S[128];
+        uint64_t idx[128] = {};
+        for (RayState& s : S) s.phase = kIdle;
+        uint64_t next = 0;
+        for (;;) {
+            int walking = 0;
+            for (const RayState& s : S) walking += s.phase == kWalk;
+            if (128 - walking >= refill || walking == 0) {
+                for (int q = 0; q < 128; q++) {
+                    enter_octants(P, S[q], &stack2[q], 128);
+                    if (is_finished(S[q].phase)) {
+                        emit_record(P, S[q], out[idx[q]]);
+                        S[q].phase = kIdle;
+                    }
+                }
+                // slot 0 of every lane first, then slot 1 (the kernel's rank order)
+                for (int j = 0; j < 2; j++)
+                    for (int l = 0; l < 64; l++) {
+                        RayState& s = S[2 * l + j];
+                        if (s.phase == kIdle && next < n) {
+                            const float* r = rays + 6 * next;
+                            idx[2 * l + j] = next++;
+                            ray_setup(s, r[0], r[1], r[2], r[3], r[4], r[5]);
+                            enter_octants(P, s, &stack2[2 * l + j], 128);
+                        }
+                    }
+                int live = 0;
+                for (const RayState& s : S) live += s.phase != kIdle;
+                if (next >= n && live == 0) break;
+            }
+            for (int st = 0; st < regroup; st++) {
+                uint32_t child[128];
+                bool w[128];
+                for (int q = 0; q < 128; q++) {
+                    w[q] = S[q].phase == kWalk;
+                    child[q] = w[q] ? walk_fetch(P, S[q]) : 0u;
+                }
+                for (int q = 0; q < 128; q++)
+                    if (w[q]) walk_step_on(P, S[q], &stack2[q], 128, child[q]);
+            }
+        }
+    } else if (!persistent) {
         for (uint64_t i = 0; i < n; i++) {
             const float* r = rays + 6 * i;
             trace_ray(P, r[0], r[1], r[2], r[3], r[4], r[5], stack.data(), 1, out[i]);

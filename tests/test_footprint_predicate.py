@@ -1,13 +1,20 @@
 """The traversal kernel replaces `float(size) / tChildExit > maxFootprint` (raytracing.cpp:276) by
 an equivalent comparison when the footprint test is disabled (maxFootprint = -1). For a size
 c = 2^k > 0 and IEEE round-to-nearest division, RN(c / t) > -1 exactly when t > 0, t = +0 or
-t < -c. Checked here against numpy's IEEE float32 division on boundary, special and random values.
+t < -c; the kernel tests the first two cases as one unsigned comparison of t's bit pattern with
+that of +inf. Checked here against numpy's IEEE float32 division on boundary, special and random values.
 """
 import numpy as np
 
 
 def fast(c, t):
-    # mirrors traverse_core.h footprint_large() for max_footprint == -1
+    # mirrors traverse_core.h footprint_large() for max_footprint == -1: the bit pattern of t is at
+    # most that of +inf (sign clear, not NaN), or t < -c
+    return (t.view(np.uint32) <= np.uint32(0x7F800000)) | (t < -c)
+
+
+def slow(c, t):
+    # the three cases spelled out
     return (t > 0) | ((t == 0) & ~np.signbit(t)) | (t < -c)
 
 
@@ -33,3 +40,6 @@ def test_disabled_footprint_predicate_matches_division():
             exact = (c / t) > np.float32(-1.0)
         assert exact.dtype == bool
         assert np.array_equal(exact, fast(c, t)), k
+        assert np.array_equal(exact, slow(c, t)), k
+        nans = np.array([0x7F800001, 0x7FC00000, 0x7FFFFFFF, 0xFF800001, 0xFFC00000], np.uint32).view(np.float32)
+        assert not fast(c, nans).any()

@@ -39,7 +39,7 @@ def _report(name, eq, rays, g, o):
     return ""
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("surf,fp", [(True, -1.0), (False, -1.0), (True, 0.05), (True, 0.0035)])
 @pytest.mark.parametrize("set_name", ["camera", "incoherent", "special"])
 def test_gpu_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, rays128, set_name, surf, fp, variant):
@@ -96,7 +96,7 @@ def _small_volumes(cq):
     }
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 def test_gpu_small_and_degenerate_volumes(cq, oracle, gpu_ctx, variant):
     from tests.raysets import special_rays
     from tests.test_oracle_pinning import _brute_rays
@@ -132,10 +132,12 @@ def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     g1 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=1))
     g3 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=3))
     g4 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=4, lds_nodes=1024))
+    g5 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=5))
     print(f"gpu {time.time() - t0:.2f}s", flush=True)
     assert records_equal(g1, g2).all()
     assert records_equal(g3, g2).all()
     assert records_equal(g4, g2).all()
+    assert records_equal(g5, g2).all()
     t0 = time.time()
     o = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, threads=16)
     print(f"oracle {time.time() - t0:.2f}s", flush=True)
@@ -160,7 +162,7 @@ def test_gpu_autotuned_launch_is_exact(cq, oracle, gpu_ctx, vol128, rays128):
     rays = rays128["incoherent"]
     r = torch.from_numpy(rays).cuda()
     best, ms = cq.autotune(ctx, dv, r, True, -1.0)
-    assert best.variant in (1, 2, 3) and ms > 0
+    assert best.variant in (1, 2, 3, 5) and ms > 0
     auto = cq.intersectVolume(ctx, dv, r, True).cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
     o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True)
     assert records_equal(auto, o).all()
