@@ -589,6 +589,7 @@ TraceParams trace_params(const cqx_dvolume* vol, const cqx_pt_params* pp, const 
     T.max_footprint = pp->max_footprint;
     T.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters;
     T.fp_disabled = footprint_is_disabled(pp->max_footprint) ? 1u : 0u;
+    T.offset32 = vol->node_count <= (uint64_t(1) << 27) ? 1u : 0u;
     return T;
 }
 

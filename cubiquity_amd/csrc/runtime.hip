@@ -335,6 +335,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
     P.max_footprint = max_footprint;
     P.max_iters = (cfg && cfg->max_iters) ? cfg->max_iters : kDefaultMaxIters;
     P.fp_disabled = footprint_is_disabled(max_footprint) ? 1u : 0u;
+    P.offset32 = vol->node_count <= (uint64_t(1) << 27) ? 1u : 0u;
 
     const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
     if (block % 64 != 0 || block == 0 || block > 256)
@@ -535,8 +536,11 @@ cqx_status cqx_volume_update(cqx_ctx* ctx, cqx_dvolume* vol, const uint32_t* nod
         HIP_TRY(hipMemcpy(vol->nodes + first * 8, nodes + first * 8, size_t(node_count - first) * 32,
                           hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(vol->sub, s, sizeof(s), hipMemcpyHostToDevice));
-    if (ctx->pt_plan && (old_nodes != vol->nodes || max_height != vol->max_height)) {
-        cqx::destroy_pt_plan(ctx->pt_plan); // its captured graph holds the old pointer / stack size
+    const uint64_t kOffset32Nodes = uint64_t(1) << 27; // the child-offset width the traversal uses
+    if (ctx->pt_plan && (old_nodes != vol->nodes || max_height != vol->max_height ||
+                         (vol->node_count <= kOffset32Nodes) != (node_count <= kOffset32Nodes))) {
+        // its captured graph holds the old pointer, stack size and child-offset width
+        cqx::destroy_pt_plan(ctx->pt_plan);
         ctx->pt_plan = nullptr;
     }
     vol->node_count = node_count;

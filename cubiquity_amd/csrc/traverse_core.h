@@ -38,6 +38,8 @@ __device__ __forceinline__ float max3f(float x, float y, float z) { return smax(
 
 __device__ __forceinline__ int find_msb(uint32_t v) { return v ? 31 - __builtin_clz(v) : -1; }
 
+// Bit k (0 <= k <= 31) of x.
+__device__ __forceinline__ uint32_t bit_at(int32_t x, int k) { return (uint32_t(x) >> k) & 1u; }
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return int32_t(uint32_t(a) + uint32_t(b)); }
 __device__ __forceinline__ int32_t wsub(int32_t a, int32_t b) { return int32_t(uint32_t(a) - uint32_t(b)); }
 __device__ __forceinline__ int32_t wmul(int32_t a, int32_t b) { return int32_t(uint32_t(a) * uint32_t(b)); }
@@ -77,6 +79,9 @@ struct TraceParams {
     float max_footprint;
     uint32_t max_iters;
     uint32_t fp_disabled; // max_footprint is exactly -1 (MAX_FOOTPRINT_DISABLED)
+    // node_count <= 2^27: every child's byte offset fits 32 bits, so child reads use a 32-bit
+    // offset from the store's base (fewer address instructions than 64-bit indexing)
+    uint32_t offset32 = 0;
 };
 
 __host__ __device__ inline bool footprint_is_disabled(float max_footprint) { return max_footprint == -1.0f; }
@@ -232,6 +237,8 @@ __device__ __forceinline__ uint32_t walk_fetch(const TraceParams& P, const RaySt
     const uint32_t slot = S.cid ^ S.sign_bits;
     if (kCache) return pick8(S.nc, slot);
     if (kTop && S.node - kMaterialCount < top_nodes) return top[(S.node - kMaterialCount) * 8 + slot];
+    if (P.offset32)
+        return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(P.nodes) + ((S.node << 5) | (slot << 2)));
     return P.nodes[uint64_t(S.node) * 8 + slot];
 }
 
@@ -303,9 +310,7 @@ __device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, 
         if (kCache) load_node(P, S.node, S.nc);
         const int32_t ns = int32_t(1u << msb);
         S.csize = ns;
-        const uint32_t bx = uint32_t(S.cpx >> msb) & 1u, by = uint32_t(S.cpy >> msb) & 1u,
-                       bz = uint32_t(S.cpz >> msb) & 1u;
-        S.cid = bx | (by << 1) | (bz << 2);
+        S.cid = bit_at(S.cpx, msb) | (bit_at(S.cpy, msb) << 1) | (bit_at(S.cpz, msb) << 2);
         // (p >> height << height) + bit msb of p * 2^msb is p with the bits below msb cleared
         // (msb <= 30 here: a crossing at bit 31 is above every SubDAG and left the walk above).
         S.cpx &= -ns;

@@ -56,6 +56,7 @@ int main(int argc, char** argv)
     P.max_footprint = float(std::atof(argv[7]));
     P.max_iters = uint32_t(std::strtoul(argv[8], nullptr, 10));
     P.fp_disabled = footprint_is_disabled(P.max_footprint) ? 1u : 0u;
+    P.offset32 = nb.size() / 32 <= (uint64_t(1) << 27) ? 1u : 0u;
     const int refill = std::atoi(argv[9]), regroup = std::atoi(argv[10]);
     const float* rays = reinterpret_cast<const float*>(rb.data());
     const uint64_t n = rb.size() / 24;
