@@ -81,7 +81,18 @@ def cpu_baseline(hv, rays, hits_gpu, cq):
         "sample": f"first {sample} of the 2^24 incoherent rays, {threads} threads, 4096-ray interleaved chunks, "
                   f"best of 3 after 1 warm-up",
         "gpu_records_bitexact_on_first": ns if same else 0,
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def timed(fn, reps, warmup=1):
@@ -189,6 +200,22 @@ def bench_edit_frames(cq, W, ctx, frames):
             "appended_nodes_per_frame": int(np.mean(appended)), "device_update_ms": mean(update_ms),
             "trace_ms_before_edits": round(before, 3), "trace_ms": mean(trace_ms),
             "frame_ms": round(float(np.mean(edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 3)}
+
+
+def bench_primary(cq, W, ctx, reps):
+    """Primary (coherent) camera rays on the 1024^3 volume: 1280x720 pixel-centre rays of the path
+    tracer's camera, intersectVolume with surface properties, the tuned launch."""
+    import torch
+
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    dv = ctx.upload(hv)
+    rays = cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, 1280, 720)
+    r = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
+    hits = torch.empty((r.shape[0], 32), dtype=torch.uint8, device="cuda")
+    ms = timed(lambda: cq.intersectVolume(ctx, dv, r, True, out=hits), reps)
+    dv.free()
+    return {"rays": int(r.shape[0]), "ms": round(ms, 4), "mrays_per_s": round(r.shape[0] / (ms * 1e-3) / 1e6, 1)}
 
 
 def bench_adjoint(cq, W, ctx, reps):
@@ -355,6 +382,8 @@ def main():
     }
     del d_hits, d_rays
     if not args.no_pathtrace:
+        out["primary_rays_1024"] = bench_primary(cq, W, ctx, max(3, args.steps // 2))
+        log(f"primary rays: {out['primary_rays_1024']}")
         out["pathtrace_1024"] = bench_pathtrace(cq, W, ctx, max(1, args.steps // 5))
         log(f"path tracing: {out['pathtrace_1024']}")
         adj = bench_adjoint(cq, W, ctx, max(2, args.steps // 4))
