@@ -67,3 +67,37 @@ def special_rays(n):
         d = d / np.linalg.norm(d)
         rays.append(o + list(d))
     return np.asarray(rays, dtype=np.float32)
+
+
+def deep_volumes(cq):
+    """Sparse volumes whose SubDAGs are 20+ levels deep: spheres about 2^20 voxels apart (edits on an
+    empty store), in one octant and across octants. Returns {name: (host volume, rays)}; the rays
+    start near one sphere or far outside and are aimed at the spheres, so that walks cross large
+    empty regions, descend deep stacks and pop back up."""
+    import numpy as np
+
+    cases = {
+        "deep_one_octant": [(3, 5, 7, 4, 2), (1 << 20, 1 << 19, 9, 6, 3), (700000, 12, 1 << 20, 5, 4)],
+        "deep_all_octants": [(-(1 << 18), 7, -(1 << 21), 6, 5), (1 << 20, -(1 << 19), 3, 3, 6), (5, 5, 5, 4, 7),
+                             (-1000, -(1 << 20), 1 << 19, 5, 8)],
+    }
+    out = {}
+    for k, (name, spheres) in enumerate(cases.items()):
+        hv = cq.HostVolume.from_dense(np.zeros((4, 4, 4), np.uint8), (0, 0, 0))
+        hv.edit_spheres([[x, y, z, r, m] for x, y, z, r, m in spheres])
+        rng = np.random.Generator(np.random.PCG64(900 + k))
+        rays = []
+        for (x, y, z, r, _) in spheres:
+            c = np.asarray([x, y, z], np.float64)
+            for (x2, y2, z2, _, _) in spheres:  # from near each sphere towards every sphere
+                o = c + rng.uniform(-3 * r, 3 * r, (60, 3))
+                t = np.asarray([x2, y2, z2], np.float64) + rng.uniform(-r, r, (60, 3))
+                d = t - o
+                rays.append(np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], axis=1))
+            o = c + rng.normal(size=(200, 3)) * 4000.0  # from far away, aimed near the sphere
+            d = c + rng.uniform(-r, r, (200, 3)) - o
+            rays.append(np.concatenate([o, d / np.linalg.norm(d, axis=1, keepdims=True)], axis=1))
+        d = rng.normal(size=(400, 3))  # random directions from the origin region
+        rays.append(np.concatenate([rng.uniform(-8, 8, (400, 3)), d / np.linalg.norm(d, axis=1, keepdims=True)], 1))
+        out[name] = (hv, np.concatenate(rays).astype(np.float32))
+    return out

@@ -70,3 +70,16 @@ def test_core_small_and_degenerate_volumes(emu, tmp_path, cq, oracle):
             got = _run(emu, tmp_path, cq, hv, rays, mode, True, -1.0, 1 << 16, 16, 4)
             ref = oracle.intersect(hv.nodes, sub, rays, True, -1.0, max_iters=1 << 16)
             assert records_equal(got, ref).all(), f"{name} {mode}"
+
+
+@pytest.mark.parametrize("mode", ["lane", "persistent", "pair"])
+def test_core_deep_sparse_volumes(emu, tmp_path, cq, oracle, mode):
+    """SubDAGs 21-22 levels deep: 23 stack rows per lane."""
+    from tests.raysets import deep_volumes
+
+    for name, (hv, rays) in deep_volumes(cq).items():
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        for surf, fp in ((True, -1.0), (True, 0.01)):
+            got = _run(emu, tmp_path, cq, hv, rays, mode, surf, fp, 1 << 16, 16, 8)
+            ref = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+            assert records_equal(got, ref).all(), f"{name} {mode} fp={fp}"

@@ -116,6 +116,22 @@ def test_gpu_small_and_degenerate_volumes(cq, oracle, gpu_ctx, variant):
         dv.free()
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_gpu_deep_sparse_volumes(cq, oracle, gpu_ctx, variant):
+    """SubDAGs 21-22 levels deep (23 stack rows per lane in LDS), every kernel variant."""
+    from tests.raysets import deep_volumes
+
+    for name, (hv, rays) in deep_volumes(cq).items():
+        dv = gpu_ctx.upload(hv)
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        for surf, fp in ((True, -1.0), (False, -1.0), (True, 0.01)):
+            o = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+            g = _gpu(gpu_ctx, dv, rays, surf, fp, cq.LaunchCfg(variant=variant, max_iters=1 << 16))
+            eq = records_equal(g, o)
+            assert eq.all(), _report(f"{name} surf={surf} fp={fp}", eq, rays, g, o)
+        dv.free()
+
+
 def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     """The benchmark workload at full size: all 2^24 incoherent rays bit-exact against the
     oracle; both kernel variants agree; hit voxels hold the hit material (Volume::voxel)."""

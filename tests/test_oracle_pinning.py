@@ -192,6 +192,25 @@ def test_small_and_degenerate_volumes_vs_reference(cq, oracle, reference, tmp_pa
             assert eq.all(), f"{name}: {(~eq).sum()} differ"
 
 
+def test_deep_sparse_volumes_vs_reference(cq, oracle, reference, tmp_path):
+    """SubDAGs 21-22 levels deep (spheres about 2^20 voxels apart): long walks through empty space,
+    deep stacks and pops back up, bit-exact against the compiled reference."""
+    from tests.raysets import deep_volumes
+
+    for name, (hv, rays) in deep_volumes(cq).items():
+        path = str(tmp_path / f"{name}.dag")
+        hv.save_dag(path)
+        rv = reference.load(path)
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        assert max(int(s["height"]) for s in sub if s["node"]) >= 21
+        for surf, fp in ((True, -1.0), (False, -1.0), (True, 0.01)):
+            h = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+            assert not ((h["info"] >> 15) & 1).any()
+            assert (h["info"] & 1).sum() > len(rays) // 3
+            eq = records_equal(oracle.to_results(rays, h, surf), rv.intersect(rays, surf, fp))
+            assert eq.all(), f"{name} surf={surf} fp={fp}: {(~eq).sum()} differ"
+
+
 # ---- closed forms ------------------------------------------------------------------------------
 
 # Direction components of exactly zero are not handled by the reference's octant walk: with
