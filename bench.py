@@ -204,7 +204,7 @@ def bench_edit_frames(cq, W, ctx, frames):
 
 def bench_primary(cq, W, ctx, reps):
     """Primary (coherent) camera rays on the 1024^3 volume: 1280x720 pixel-centre rays of the path
-    tracer's camera, intersectVolume with surface properties, the tuned launch."""
+    tracer's camera, intersectVolume with surface properties, the launch tuned for this batch."""
     import torch
 
     w = W.PATHTRACE_1024
@@ -213,9 +213,11 @@ def bench_primary(cq, W, ctx, reps):
     rays = cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, 1280, 720)
     r = torch.from_numpy(np.ascontiguousarray(rays)).cuda()
     hits = torch.empty((r.shape[0], 32), dtype=torch.uint8, device="cuda")
+    best, _ = cq.autotune(ctx, dv, r, True)  # this batch-size class has its own tuned launch
     ms = timed(lambda: cq.intersectVolume(ctx, dv, r, True, out=hits), reps)
     dv.free()
-    return {"rays": int(r.shape[0]), "ms": round(ms, 4), "mrays_per_s": round(r.shape[0] / (ms * 1e-3) / 1e6, 1)}
+    return {"rays": int(r.shape[0]), "ms": round(ms, 4), "mrays_per_s": round(r.shape[0] / (ms * 1e-3) / 1e6, 1),
+            "launch": {f: getattr(best, f) for f, _ in best._fields_}}
 
 
 def bench_adjoint(cq, W, ctx, reps):

@@ -76,9 +76,10 @@ class HostVolume:
     def save_dag(self, path):
         check(lib().cqx_hv_save_dag(self._h, os.fsencode(path)))
 
-    def edit_spheres(self, edits):
+    def edit_spheres(self, edits, threads=0):
         """Sphere fill/carve edits (fillBrush with SphereBrush semantics). `edits`: (n, 5) array of
-        centre x, y, z, radius, material (0 carves). Returns the index of the first appended node."""
+        centre x, y, z, radius, material (0 carves); `threads`: 0 all cores, 1 in order on this
+        thread. Returns the index of the first appended node."""
         e = np.asarray(edits, dtype=np.float64).reshape(-1, 5)
         arr = np.zeros(max(1, len(e)), dtype=SPHERE_EDIT_DTYPE)
         arr["centre"][: len(e)] = e[:, :3]
@@ -87,8 +88,8 @@ class HostVolume:
             raise CqxError("material out of range")
         arr["material"][: len(e)] = e[:, 4].astype(np.uint64)
         first = ctypes.c_uint64()
-        check(lib().cqx_hv_edit_spheres(self._h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(e)),
-                                        ctypes.byref(first)))
+        check(lib().cqx_hv_edit_spheres_mt(self._h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(e)),
+                                           ctypes.c_int32(threads), ctypes.byref(first)))
         return int(first.value)
 
     def checkpoint(self):

@@ -102,6 +102,7 @@ SIGNATURES = {
     "cqx_hv_generate": (_ST, [_P, ctypes.c_int, _P]),
     "cqx_hv_from_dense": (_ST, [_P, _P, _P, _P]),
     "cqx_hv_edit_spheres": (_ST, [_P, _P, _U64, _P]),
+    "cqx_hv_edit_spheres_mt": (_ST, [_P, _P, _U64, ctypes.c_int32, _P]),
     "cqx_hv_compact": (_ST, [_P]),
     "cqx_hv_checkpoint": (_ST, [_P]),
     "cqx_materials_load": (_ST, [ctypes.c_char_p, _P, _P, _P, _P]),

@@ -17,10 +17,13 @@
 // appended range only.
 #include "cqx_internal.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <limits>
 #include <new>
+#include <thread>
 
 namespace cqx {
 namespace {
@@ -53,8 +56,47 @@ struct Interner {
             slots.assign(cap, kEmpty);
             indexed = kMaterialCount;
         }
-        for (size_t j = indexed; j < nodes.size(); j++) insert_slot(uint32_t(j));
+        if (nodes.size() - indexed >= (size_t(1) << 20)) insert_parallel(indexed, nodes.size());
+        else
+            for (size_t j = indexed; j < nodes.size(); j++) insert_slot(uint32_t(j));
         indexed = nodes.size();
+    }
+
+    // insert_slot for a large range on all threads: a slot goes from empty to an index with a
+    // compare-and-swap, and where two nodes are equal the lower index wins (an atomic minimum),
+    // so lookups return what the serial insertion would.
+    void insert_parallel(size_t from, size_t to)
+    {
+        const size_t mask = slots.size() - 1;
+        uint32_t* const sl = slots.data();
+        std::atomic<size_t> next{ from };
+        auto work = [&]() {
+            constexpr size_t kBatch = 4096;
+            for (size_t b; (b = next.fetch_add(kBatch)) < to;)
+                for (size_t j = b; j < std::min(to, b + kBatch); j++) {
+                    for (size_t i = hash_node(nodes[j]) & mask;; i = (i + 1) & mask) {
+                        uint32_t cur = __atomic_load_n(&sl[i], __ATOMIC_ACQUIRE);
+                        if (cur == kEmpty) {
+                            if (__atomic_compare_exchange_n(&sl[i], &cur, uint32_t(j), false, __ATOMIC_ACQ_REL,
+                                                            __ATOMIC_ACQUIRE))
+                                break;
+                            // another thread took the slot: look at what it holds
+                        }
+                        if (std::memcmp(&nodes[cur], &nodes[j], sizeof(Node)) == 0) {
+                            while (uint32_t(j) < cur &&
+                                   !__atomic_compare_exchange_n(&sl[i], &cur, uint32_t(j), false, __ATOMIC_ACQ_REL,
+                                                                __ATOMIC_ACQUIRE)) {
+                            }
+                            break;
+                        }
+                    }
+                }
+        };
+        const int nt = default_threads();
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
     }
 
     void insert_slot(uint32_t j)
@@ -66,6 +108,16 @@ struct Interner {
                 return;
             }
             if (std::memcmp(&nodes[slots[i]], &nodes[j], sizeof(Node)) == 0) return; // keep the first
+        }
+    }
+
+    // Index of a node with this content, or kEmpty (read-only: safe while no thread interns).
+    uint32_t find(const Node& n) const
+    {
+        const size_t mask = slots.size() - 1;
+        for (size_t i = hash_node(n) & mask;; i = (i + 1) & mask) {
+            const uint32_t s = slots[i];
+            if (s == kEmpty || std::memcmp(&nodes[s], &n, sizeof(Node)) == 0) return s;
         }
     }
 
@@ -124,20 +176,99 @@ struct SphereBrushF {
     }
 };
 
-struct EditWalk {
+// Node access and interning for an edit walk over the whole store (sequential edits).
+struct GlobalStore {
     const std::vector<Node>& nodes;
     Interner& in;
+    uint32_t child(uint32_t node, uint32_t c) const { return node < kMaterialCount ? node : nodes[node].c[c]; }
+    uint32_t intern(const Node& n) { return in.intern(n); }
+};
+
+// The nodes one region creates during parallel edits: a private list whose entries have the
+// temporary indices base + i (base = store size when the edits began), over a read-only view of
+// the store. A node equal to one already in the store is not created; merge() moves the list into
+// the store afterwards.
+struct RegionStore {
+    const std::vector<Node>& nodes;
+    const Interner& in;
+    uint32_t base;
+    std::vector<Node> local;
+    std::vector<uint32_t> slots; // open addressing over `local`
+    static constexpr uint32_t kEmpty = 0xffffffffu;
+
+    uint32_t child(uint32_t node, uint32_t c) const
+    {
+        if (node < kMaterialCount) return node;
+        return node >= base ? local[node - base].c[c] : nodes[node].c[c];
+    }
+
+    uint32_t intern(const Node& n)
+    {
+        bool uniform = n.c[0] < kMaterialCount; // the same collapse rule as Interner::intern
+        bool has_local = false;
+        for (int i = 0; i < 8; i++) {
+            uniform = uniform && n.c[i] == n.c[0];
+            has_local = has_local || n.c[i] >= base;
+        }
+        if (uniform) return n.c[0];
+        if (!has_local) {
+            const uint32_t g = in.find(n);
+            if (g != Interner::kEmpty) return g;
+        }
+        if (2 * (local.size() + 1) > slots.size()) {
+            slots.assign(std::max<size_t>(64, slots.size() * 2), kEmpty);
+            for (size_t j = 0; j < local.size(); j++) insert_slot(uint32_t(j));
+        }
+        const size_t mask = slots.size() - 1;
+        for (size_t i = hash_node(n) & mask;; i = (i + 1) & mask) {
+            const uint32_t s = slots[i];
+            if (s == kEmpty) {
+                if (uint64_t(base) + local.size() >= 0xfffffffeull) throw std::bad_alloc();
+                slots[i] = uint32_t(local.size());
+                local.push_back(n);
+                return base + slots[i];
+            }
+            if (std::memcmp(&local[s], &n, sizeof(Node)) == 0) return base + s;
+        }
+    }
+
+    void insert_slot(uint32_t j)
+    {
+        const size_t mask = slots.size() - 1;
+        for (size_t i = hash_node(local[j]) & mask;; i = (i + 1) & mask)
+            if (slots[i] == kEmpty) {
+                slots[i] = j;
+                return;
+            }
+    }
+
+    // Interns the list into the store (children before parents, as created) and returns the
+    // store index of `node` (a store index, or a temporary one of this list).
+    uint32_t merge(Interner& global, uint32_t node)
+    {
+        std::vector<uint32_t> map(local.size());
+        for (size_t i = 0; i < local.size(); i++) {
+            Node n = local[i];
+            for (int c = 0; c < 8; c++)
+                if (n.c[c] >= base) n.c[c] = map[n.c[c] - base];
+            map[i] = global.intern(n);
+        }
+        return node >= base ? map[node - base] : node;
+    }
+};
+
+template <typename Store>
+struct EditWalk {
+    Store& st;
     const SphereBrushF& b;
     uint32_t mat;
-
-    uint32_t child_of(uint32_t node, uint32_t c) const { return node < kMaterialCount ? node : nodes[node].c[c]; }
 
     // fillBrush(volume, brush, matId, nodeIndex, nodeHeight, lower), voxelization.cpp:837-915
     uint32_t fill(uint32_t node, int height, int32_t lx, int32_t ly, int32_t lz)
     {
         const uint32_t ch = uint32_t(height - 1);
         Node out;
-        for (int c = 0; c < 8; c++) out.c[c] = child_of(node, uint32_t(c));
+        for (int c = 0; c < 8; c++) out.c[c] = st.child(node, uint32_t(c));
         bool changed = false;
         for (uint32_t cz = 0; cz <= 1; cz++)
             for (uint32_t cy = 0; cy <= 1; cy++)
@@ -168,9 +299,82 @@ struct EditWalk {
                     }
                 }
         if (!changed) return node;
-        return in.intern(out);
+        return st.intern(out);
     }
 };
+
+// ---- parallel edits -----------------------------------------------------------------------------
+// The volume is cut into aligned cubes ("regions") of side 2^hs, with hs chosen so that no node
+// of height >= hs fits inside any of the spheres: (2^hs - 1) * sqrt(3) > 2 r + 4. Above the
+// regions fillBrush then only descends (no node there can have its 8 corners inside a sphere),
+// and the float box overlap test is monotone in the integer corners, so applying each sphere, in
+// order, to every region its bounds touch and rebuilding the levels above gives the voxels of the
+// sequential walk. Regions are independent: each applies its spheres in order into a private node
+// list (RegionStore), in parallel; the lists are merged into the store in region order, so the
+// result does not depend on the number of threads.
+
+struct RegionKey {
+    uint32_t x, y, z; // region index along each axis of the offset coordinates u = v + 2^31
+    bool operator<(const RegionKey& o) const { return x != o.x ? x < o.x : y != o.y ? y < o.y : z < o.z; }
+    bool operator==(const RegionKey& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+
+// Smallest region height for these edits (>= 5), or -1 for sequential edits: spheres too large,
+// or bounds not within 2^23 voxels of the origin (there every voxel coordinate the walk converts
+// to float is exact, so a region's float box overlaps a sphere's bounds exactly when its integer
+// box meets them).
+int region_height(const cqx_sphere_edit* edits, uint64_t n)
+{
+    double rmax = 0.0;
+    for (uint64_t i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            const double c = edits[i].centre[a];
+            if (!(std::fabs(c) + edits[i].radius < 8388608.0)) return -1;
+        }
+        rmax = std::max(rmax, double(edits[i].radius));
+    }
+    for (int hs = 5; hs <= 20; hs++)
+        if ((std::ldexp(1.0, hs) - 1.0) * std::sqrt(3.0) > 2.0 * rmax + 4.0) return hs;
+    return -1;
+}
+
+// The node at height hs whose region is `k`, found by descending from the root (Volume::voxel's
+// walk, storage.cpp:481-499).
+uint32_t region_node(const std::vector<Node>& nodes, uint32_t root, int hs, const RegionKey& k)
+{
+    uint32_t node = root;
+    const uint32_t ux = k.x << hs, uy = k.y << hs, uz = k.z << hs;
+    for (int h = kRootHeight; h > hs && node >= kMaterialCount; h--) {
+        const uint32_t c = ((ux >> (h - 1)) & 1u) | (((uy >> (h - 1)) & 1u) << 1) | (((uz >> (h - 1)) & 1u) << 2);
+        node = nodes[node].c[c];
+    }
+    return node;
+}
+
+// Rebuilds the levels above the regions: `regs` indexes the edited regions inside this node.
+uint32_t rebuild_top(GlobalStore& g, uint32_t node, int h, int hs, const std::vector<RegionKey>& keys,
+                     const std::vector<uint32_t>& results, const std::vector<uint32_t>& regs)
+{
+    if (h == hs) return results[regs[0]];
+    std::vector<uint32_t> bucket[8];
+    for (uint32_t r : regs) {
+        const uint32_t ux = keys[r].x << hs, uy = keys[r].y << hs, uz = keys[r].z << hs;
+        const uint32_t c = ((ux >> (h - 1)) & 1u) | (((uy >> (h - 1)) & 1u) << 1) | (((uz >> (h - 1)) & 1u) << 2);
+        bucket[c].push_back(r);
+    }
+    Node out;
+    for (int c = 0; c < 8; c++) out.c[c] = g.child(node, uint32_t(c));
+    bool changed = false;
+    for (int c = 0; c < 8; c++) {
+        if (bucket[c].empty()) continue;
+        const uint32_t nc = rebuild_top(g, out.c[c], h - 1, hs, keys, results, bucket[c]);
+        if (nc != out.c[c]) {
+            out.c[c] = nc;
+            changed = true;
+        }
+    }
+    return changed ? g.intern(out) : node;
+}
 
 } // namespace
 } // namespace cqx
@@ -179,7 +383,8 @@ using namespace cqx;
 
 extern "C" {
 
-cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n, uint64_t* first_new)
+cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n, int32_t threads,
+                                  uint64_t* first_new)
 {
     if (!hv || (n && !edits)) return fail(CQX_ERR_INVALID_ARG, "null argument");
     if (first_new) *first_new = hv->v.nodes.size();
@@ -189,19 +394,94 @@ cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits
                                                               (unsigned long long)i);
         if (!(edits[i].radius >= 0.0f)) return fail(CQX_ERR_INVALID_ARG, "edit %llu: bad radius", (unsigned long long)i);
     }
+    if (threads <= 0) threads = default_threads();
+    std::vector<Node>& nodes = hv->v.nodes;
     try {
-        Interner in(hv->v.nodes, hv->v.edit_slots, hv->v.edit_indexed);
-        uint32_t root = hv->v.root;
-        for (uint64_t i = 0; i < n; i++) {
-            const SphereBrushF b(edits[i]);
-            EditWalk w{ hv->v.nodes, in, b, edits[i].material };
-            root = w.fill(root, kRootHeight, INT32_MIN, INT32_MIN, INT32_MIN);
+        Interner in(nodes, hv->v.edit_slots, hv->v.edit_indexed);
+        GlobalStore g{ nodes, in };
+        const int hs = (threads > 1 && n >= 16) ? region_height(edits, n) : -1;
+        if (hs < 0) {
+            uint32_t root = hv->v.root;
+            for (uint64_t i = 0; i < n; i++) {
+                const SphereBrushF b(edits[i]);
+                EditWalk<GlobalStore> w{ g, b, edits[i].material };
+                root = w.fill(root, kRootHeight, INT32_MIN, INT32_MIN, INT32_MIN);
+            }
+            hv->v.root = root;
+            return CQX_OK;
         }
-        hv->v.root = root;
+        // Regions each edit's bounds touch (one voxel of margin), edits kept in order per region.
+        std::vector<std::pair<RegionKey, uint32_t>> touch;
+        for (uint64_t i = 0; i < n; i++) {
+            uint32_t lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                const SphereBrushF b(edits[i]);
+                const int64_t vlo = int64_t(std::floor(double(b.lo[a]))) - 1, vhi = int64_t(std::ceil(double(b.hi[a]))) + 1;
+                lo[a] = uint32_t(std::max<int64_t>(vlo, INT32_MIN) + (int64_t(1) << 31)) >> hs;
+                hi[a] = uint32_t(std::min<int64_t>(vhi, INT32_MAX) + (int64_t(1) << 31)) >> hs;
+            }
+            for (uint32_t z = lo[2]; z <= hi[2]; z++)
+                for (uint32_t y = lo[1]; y <= hi[1]; y++)
+                    for (uint32_t x = lo[0]; x <= hi[0]; x++) touch.push_back({ RegionKey{ x, y, z }, uint32_t(i) });
+        }
+        std::stable_sort(touch.begin(), touch.end(),
+                         [](const auto& a, const auto& b) { return a.first < b.first; });
+        std::vector<RegionKey> keys;
+        std::vector<size_t> first; // first entry of each region in `touch`
+        for (size_t j = 0; j < touch.size(); j++)
+            if (j == 0 || !(touch[j].first == touch[j - 1].first)) {
+                keys.push_back(touch[j].first);
+                first.push_back(j);
+            }
+        first.push_back(touch.size());
+        const size_t nr = keys.size();
+        const uint32_t base = uint32_t(nodes.size());
+        std::vector<RegionStore> stores;
+        stores.reserve(nr);
+        for (size_t r = 0; r < nr; r++) stores.push_back(RegionStore{ nodes, in, base, {}, {} });
+        std::vector<uint32_t> results(nr);
+        std::atomic<size_t> next{ 0 };
+        std::atomic<bool> oom{ false };
+        auto work = [&]() {
+            try {
+                for (size_t r; (r = next.fetch_add(1)) < nr;) {
+                    const RegionKey& k = keys[r];
+                    uint32_t node = region_node(nodes, hv->v.root, hs, k);
+                    const int32_t lx = int32_t((k.x << hs) - 0x80000000u), ly = int32_t((k.y << hs) - 0x80000000u),
+                                  lz = int32_t((k.z << hs) - 0x80000000u);
+                    for (size_t j = first[r]; j < first[r + 1]; j++) {
+                        const cqx_sphere_edit& e = edits[touch[j].second];
+                        const SphereBrushF b(e);
+                        EditWalk<RegionStore> w{ stores[r], b, e.material };
+                        node = w.fill(node, hs, lx, ly, lz);
+                    }
+                    results[r] = node;
+                }
+            } catch (const std::bad_alloc&) {
+                oom = true;
+                next = nr;
+            }
+        };
+        const int nt = int(std::min<size_t>(size_t(threads), nr));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(work);
+        work();
+        for (std::thread& t : pool) t.join();
+        if (oom) throw std::bad_alloc();
+        for (size_t r = 0; r < nr; r++) results[r] = stores[r].merge(in, results[r]);
+        stores.clear();
+        std::vector<uint32_t> all(nr);
+        for (size_t r = 0; r < nr; r++) all[r] = uint32_t(r);
+        hv->v.root = rebuild_top(g, hv->v.root, kRootHeight, hs, keys, results, all);
         return CQX_OK;
     } catch (const std::bad_alloc&) {
         return fail(CQX_ERR_OOM, "out of memory while editing");
     }
+}
+
+cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n, uint64_t* first_new)
+{
+    return cqx_hv_edit_spheres_mt(hv, edits, n, 0, first_new);
 }
 
 cqx_status cqx_hv_compact(cqx_host_volume* hv)

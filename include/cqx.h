@@ -175,6 +175,14 @@ typedef struct {
 
 cqx_status cqx_hv_edit_spheres(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n,
                                uint64_t* first_new);
+/* The same with an explicit number of host threads (0 = all cores or OMP_NUM_THREADS, 1 = in
+ * order on the calling thread). With more than one thread and at least 16 edits, the volume is
+ * cut into aligned regions too large to lie inside any of the spheres; each region applies its
+ * spheres in order, regions in parallel, and the levels above are rebuilt once. The voxels are
+ * those of the in-order walk for any thread count (spheres farther than 2^23 voxels from the
+ * origin are applied in order). */
+cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* edits, uint64_t n,
+                                  int32_t threads, uint64_t* first_new);
 /* Drops unreachable nodes and renumbers breadth-first (what Volume::bake does for storage,
  * storage.cpp:387-394). Invalidates device copies (upload again). */
 cqx_status cqx_hv_compact(cqx_host_volume* hv);

@@ -174,3 +174,52 @@ def test_gpu_traversal_after_undo_redo(cq, oracle, gpu_ctx):
         ref = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, threads=8)
         assert records_equal(got, ref).all(), op
     dv.free()
+
+
+def _store_digest(hv):
+    import hashlib
+
+    return hashlib.sha256(hv.nodes.tobytes()).hexdigest(), hv.root
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_parallel_edits_equal_in_order_edits(cq, oracle, seed):
+    """Region-parallel edits (any thread count) give the voxels of the in-order walk, the same
+    canonical DAG after compaction, and a store that does not depend on the number of threads."""
+    p = cq.gen_default_params(7, seed, n_materials=9)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = np.zeros((1200, 5))
+    e[:, :3] = rng.uniform(-12.0, 140.0, (1200, 3))
+    e[:, 3] = rng.uniform(0.5, 12.0, 1200)
+    e[:, 4] = rng.integers(0, 10, 1200)
+    e[::5, :3] = e[1::5, :3] + rng.uniform(-3, 3, (240, 3))  # overlapping pairs, applied in order
+    seq = cq.HostVolume.generate(p)
+    seq.edit_spheres(e, threads=1)
+    stores = []
+    for t in (2, 3, 8):
+        par = cq.HostVolume.generate(p)
+        par.edit_spheres(e, threads=t)
+        stores.append(_store_digest(par))
+    assert stores[0] == stores[1] == stores[2]
+    lo, size = (-16, -16, -16), (160, 160, 160)
+    assert np.array_equal(oracle.dense(seq.nodes, seq.root, lo, size), oracle.dense(par.nodes, par.root, lo, size))
+    seq.compact()
+    par.compact()
+    assert _store_digest(seq) == _store_digest(par)
+
+
+def test_parallel_edits_fall_back_for_far_or_large_spheres(cq, oracle):
+    """Spheres beyond 2^23 voxels from the origin, or too large for any region, are applied in
+    order; mixed batches still give the in-order store."""
+    base = cq.gen_default_params(6, 5)
+    far = np.asarray([[9.0e6, 5.0, 5.0, 6.0, 3.0], [-9.1e6, -4.0e6, 3.0, 5.0, 4.0]] +
+                     [[10.0 + i, 20.0, 30.0, 3.0, 5.0] for i in range(20)])
+    # a carve of radius 3e6 empties the volume cheaply (empty children already hold material 0)
+    big = np.asarray([[32.0, 32.0, 32.0, 3.0e6, 0.0]] + [[5.0 * i, 7.0, 9.0, 2.0, 6.0] for i in range(20)])
+    for edits in (far, big):
+        a = cq.HostVolume.generate(base)
+        b = cq.HostVolume.generate(base)
+        a.edit_spheres(edits, threads=1)
+        b.edit_spheres(edits, threads=8)
+        assert _store_digest(a) == _store_digest(b)
+    assert b.voxel(40, 40, 40) == 0 and b.voxel(5, 7, 9) == 6
