@@ -272,8 +272,8 @@ def main():
                          "kernel for calibrating rocprof's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays shard by "
-              f"replicating the node store per GPU; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (rays would shard by "
+              f"replicating the read-only node store on each GPU and splitting the ray batches)", file=sys.stderr)
         sys.exit(2)
 
     import torch
