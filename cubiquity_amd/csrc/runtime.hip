@@ -57,6 +57,24 @@ __global__ void k_reset_counters(unsigned long long* counters)
     if (threadIdx.x < kQueues) counters[threadIdx.x * kCounterStride] = 0;
 }
 
+// Rays are read once and records written once: streaming (non-temporal) accesses keep them from
+// displacing node lines in L2.
+__device__ __forceinline__ void setup_from(RayState& S, const float* r)
+{
+    float v[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) v[i] = __builtin_nontemporal_load(r + i);
+    ray_setup(S, v[0], v[1], v[2], v[3], v[4], v[5]);
+}
+
+__device__ __forceinline__ void store_record(cqx_hit* dst, const cqx_hit& h)
+{
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(&h);
+    uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+#pragma unroll
+    for (int i = 0; i < 8; i++) __builtin_nontemporal_store(s[i], d + i);
+}
+
 // Where a persistent wave's ray ids come from. SingleSource: one volume, one ray array; its
 // parameters stay wave-uniform. BatchSource: many (volume, ray array) batches laid end to end in
 // one id space; a lane finds its batch by binary search over the batches' first ids.
@@ -160,7 +178,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
             if (is_finished(S.phase)) {
                 cqx_hit out;
                 emit_record(uniform_params(src, P), S, out);
-                *dst = out;
+                store_record(dst, out);
                 S.phase = kIdle;
             }
             // Claim a new chunk once the current one is used up (at most one claim per queue
@@ -187,7 +205,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                 if (k < c_end) {
                     const float* r;
                     src.fetch(k, P, r, dst);
-                    ray_setup(S, r[0], r[1], r[2], r[3], r[4], r[5]);
+                    setup_from(S, r);
                     enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
                 }
             }
@@ -241,7 +259,7 @@ __global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64
                 if (is_finished(S[j].phase)) {
                     cqx_hit out;
                     emit_record(P, S[j], out);
-                    *dst[j] = out;
+                    store_record(dst[j], out);
                     S[j].phase = kIdle;
                 }
             }
@@ -273,7 +291,7 @@ __global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64
                     if (k < c_end) {
                         const float* r;
                         src.fetch(k, unused, r, dst[j]);
-                        ray_setup(S[j], r[0], r[1], r[2], r[3], r[4], r[5]);
+                        setup_from(S[j], r);
                         enter_octants(P, S[j], stk[j], stride);
                     }
                 }
