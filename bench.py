@@ -220,6 +220,37 @@ def bench_primary(cq, W, ctx, reps):
             "launch": {f: getattr(best, f) for f, _ in best._fields_}}
 
 
+def bench_scale(cq, W, ctx, reps):
+    """The traversal on larger volumes of the headline family (2^24 incoherent rays each, tuned
+    launch): node store size, Mrays/s and algorithmic bandwidth, from MALL- to HBM-resident."""
+    import torch
+
+    out = []
+    for log2 in (12, 13):
+        w = W.scaled_headline(log2)
+        t0 = time.time()
+        hv = cq.HostVolume.generate(w.params())
+        build_s = time.time() - t0
+        r = torch.from_numpy(cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)).cuda()
+        hits = torch.empty((r.shape[0], 32), dtype=torch.uint8, device="cuda")
+        dv = ctx.upload(hv)
+        best, _ = cq.autotune(ctx, dv, r, True)
+        ms = timed(lambda: cq.intersectVolume(ctx, dv, r, True, out=hits), reps)
+        h = hits.cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
+        entries = int(h["entries"].astype(np.int64).sum())
+        alg = r.shape[0] * (24 + 32) + 32 * entries
+        out.append({"side": w.side, "nodes": hv.node_count, "store_gb": round(hv.node_count * 32 / 1e9, 3),
+                    "build_s": round(build_s, 1), "ms": round(ms, 3),
+                    "mrays_per_s": round(r.shape[0] / (ms * 1e-3) / 1e6, 1),
+                    "node_entries_per_ray": round(entries / r.shape[0], 2),
+                    "algorithmic_gb_per_s": round(alg / (ms * 1e-3) / 1e9, 1),
+                    "hit_fraction": round(float((h["info"] & 1).mean()), 4),
+                    "launch": {f: getattr(best, f) for f, _ in best._fields_}})
+        dv.free()
+        del hv, r, hits
+    return out
+
+
 def bench_adjoint(cq, W, ctx, reps):
     """Differentiable rendering (512^3, 16 materials, 512x512, 8 spp, 3 bounces): forward time,
     adjoint (path replay) time and their ratio."""
@@ -265,6 +296,10 @@ def main():
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--scale", action="store_true",
+                    help="also time the traversal on the 4096^3 and 8192^3 volumes of the same family")
+    ap.add_argument("--log2-size", type=int, default=10,
+                    help="side of the traversal volume (for profiling at other sizes; the metric is quoted at 10)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch defaults)")
     ap.add_argument("--launch", default="", help="explicit launch, e.g. variant=2,waves_per_cu=20,regroup_interval=8")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -281,7 +316,7 @@ def main():
     import cubiquity_amd as cq
     from cubiquity_amd import workloads as W
 
-    w = W.PATHTRACE_1024
+    w = W.PATHTRACE_1024 if args.log2_size == 10 else W.scaled_headline(args.log2_size)
     t0 = time.time()
     hv = cq.HostVolume.generate(w.params())
     rays = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, args.rays)
@@ -339,7 +374,7 @@ def main():
     anomalies = int(((hits["info"] >> 15) & 1).sum())
     alg_bytes = args.rays * (24 + 32) + 32 * entries
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic, l2_hit_rate = traffic_from_profiles()
+    traffic, l2_hit_rate = traffic_from_profiles() if args.log2_size == 10 else (None, None)
 
     value = args.rays * args.steps / wall / 1e6
     out = {
@@ -356,7 +391,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "pathtrace_1024 volume (1024^3, seed 1) + 2^24 incoherent rays (seed 7), "
+            "workload": f"{w.name} volume ({w.side}^3, seed 1) + {args.rays} incoherent rays (seed 7), "
                         "intersectVolume surface properties on, LOD off",
             "rays": args.rays,
             "nodes": hv.node_count,
@@ -395,6 +430,9 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(cq, W, ctx, max(2, args.steps // 4))
         log(f"batched: {out['batched']}")
+    if args.scale:
+        out["scale"] = bench_scale(cq, W, ctx, args.steps)
+        log(f"scale: {out['scale']}")
     if not args.no_edits:
         out["edit_frames_2048"] = bench_edit_frames(cq, W, ctx, 3)
         log(f"edit frames: {out['edit_frames_2048']}")

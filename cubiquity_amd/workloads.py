@@ -56,6 +56,13 @@ DIFFRENDER_512 = Workload("diffrender_512", 9, 5, {"n_materials": 16}, _camera(9
 LARGE_16384 = Workload("large_16384", 14, 9, {"n_materials": 255, "n_spheres": 1 << 20}, None, 7, 1 << 26)
 
 
+# The headline volume family at larger sides (same generator, seed 1, 9 materials; default sphere
+# counts and radii scale with the side): 4096^3 holds 11.6M nodes (0.37 GB), 8192^3 57.4M nodes
+# (1.84 GB), past the 256 MB MALL, so traversal there reads its nodes from HBM.
+def scaled_headline(log2_size):
+    return Workload(f"pathtrace_family_{1 << log2_size}", log2_size, 1, {"n_materials": 9}, None, 7, 1 << 24)
+
+
 # Edit frames (a scaled form of the 16384^3 / 1M-sphere / 10k-edits-per-frame workload, whose
 # volume does not fit a 64 GB build host): a 2048^3 volume, then per frame 1000 sphere edits
 # (seeded by frame) applied on the host and uploaded as an append-only delta, then 2^24
