@@ -67,8 +67,8 @@ def cpu_baseline(hv, rays, hits_gpu, cq):
             check = lambda r: orc.to_results(r, orc.intersect(hv.nodes, sub, r, True, threads=threads))
         t_cal, _ = timer(rays[: 1 << 16], 1)
         rate = (1 << 16) / max(t_cal, 1e-9)
-        sample = int(min(rays.shape[0], max(1 << 18, rate * 3.0)))  # ~3 s per repetition
-        t, nhit = timer(rays[:sample], 3)
+        sample = int(min(rays.shape[0], max(1 << 18, rate * 2.0)))  # ~2 s per repetition
+        t, nhit = timer(rays[:sample], 5)
         ns = 1 << 15
         ref = check(rays[:ns])
         mine = cq.hits_to_intersections(rays[:ns], hits_gpu[:ns], True)
@@ -79,7 +79,7 @@ def cpu_baseline(hv, rays, hits_gpu, cq):
         "cores": threads,
         "kind": kind,
         "sample": f"first {sample} of the 2^24 incoherent rays, {threads} threads, 4096-ray interleaved chunks, "
-                  f"best of 3 after 1 warm-up",
+                  f"best of 5 after 1 warm-up",
         "gpu_records_bitexact_on_first": ns if same else 0,
         "cpu_model": cpu_model(),
     }
