@@ -268,13 +268,18 @@ cqx_status cqx_volume_update(cqx_ctx* ctx, cqx_dvolume* vol, const uint32_t* nod
 cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, uint64_t* node_count,
                            uint32_t* root, cqx_subdag sub[8]);
 
-/* Launch parameters of the traversal kernel. Zero fields take the tuned or built-in defaults. */
+/* Launch parameters of the traversal kernel. Zero fields take the tuned or built-in defaults
+ * (variant 2, 256 threads, 32 waves per CU, regroup 8, refill 16). */
 typedef struct {
-    uint32_t variant;          /* 0 auto, 1 one ray per lane, 2 persistent with wave refill, 3 = 2 + node cache */
-    uint32_t block;            /* threads per workgroup (multiple of 64) */
+    uint32_t variant;          /* 0 auto (the autotuner's choice), 1 one ray per lane, 2 persistent waves
+                                  (walk steps + service passes, rays claimed in chunks), 3 = 2 with the
+                                  current node's children in registers, 4 = 2 with the top nodes in
+                                  LDS, 5 = 2 with two rays per lane */
+    uint32_t block;            /* threads per workgroup (multiple of 64, at most 256) */
     uint32_t waves_per_cu;     /* persistent grid size: CUs x this many waves */
-    uint32_t regroup_interval; /* persistent: traversal steps between refill checks */
-    uint32_t refill_lanes;     /* persistent: refill once at least this many lanes are idle */
+    uint32_t regroup_interval; /* persistent: traversal steps between service-pass checks */
+    uint32_t refill_lanes;     /* persistent: run a service pass once at least this many lanes (ray
+                                  slots for variant 5) wait: finished, between SubDAGs or idle */
     uint32_t max_iters;        /* per-ray iteration cap (0 = built-in 1<<22) */
     uint32_t lds_nodes;        /* variant 4: internal nodes [256, 256 + lds_nodes) read from a copy in
                                   LDS (the top of a breadth-first store; 0 = 512) */
