@@ -17,6 +17,7 @@ from .cabi import (CQX_SURFACE_PROPERTIES, MAX_FOOTPRINT_DISABLED, CqxError, Gen
 
 __all__ = [
     "Materials",
+    "traceRayRef",
     "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
     "incoherent_rays", "gen_default_params", "hits_to_intersections", "CqxError", "GenParams",
     "LaunchCfg", "HIT_DTYPE", "INTERSECTION_DTYPE", "SUBDAG_DTYPE", "CQX_SURFACE_PROPERTIES",
@@ -296,6 +297,21 @@ def autotune(ctx, dvol, rays, computeSurfaceProperties=True, maxFootprint=MAX_FO
                              ctypes.c_float(maxFootprint), ctypes.c_void_p(hits.data_ptr()), ctypes.c_void_p(stream),
                              ctypes.byref(best), ctypes.byref(ms)))
     return best, float(ms.value)
+
+
+def traceRayRef(ctx, dvol, rays, stream=None):
+    """The reference's brute-force raycast (cqx_trace_ref) on GPU rays (n, 6) float32; returns a
+    uint8 GPU tensor of n 40-byte RayVolumeIntersection records (INTERSECTION_DTYPE)."""
+    import torch
+
+    rays = rays.contiguous().view(-1, 6)
+    n = rays.shape[0]
+    out = torch.empty((n, INTERSECTION_DTYPE.itemsize), dtype=torch.uint8, device=rays.device)
+    if stream is None:
+        stream = torch.cuda.current_stream(rays.device).cuda_stream
+    check(lib().cqx_trace_ref(ctx._h, dvol._h, ctypes.c_void_p(rays.data_ptr()), ctypes.c_uint64(n),
+                              ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+    return out
 
 
 def intersect_batched(ctx, items, cfg=None, stream=None):

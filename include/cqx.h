@@ -318,6 +318,15 @@ typedef struct {
 cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_t n_batches,
                                  const cqx_launch_cfg* cfg, void* stream);
 
+/* traceRayRef (src/library/raytracing.cpp:480-535 with visitVolumeNodes, utility.h:60-116): the
+ * reference's brute-force raycast, one ray per GPU lane. Depth-first over every node whose box
+ * dilated by 0.5 the ray pierces (FP64 slab test, entry and exit stored as float); the hit is the
+ * first non-empty material node with the smallest entry among those with exit > 0. Fills hit,
+ * material and distance (DBL_MAX on a miss); position and normal are 0, as the reference leaves
+ * them. rays and out are device pointers. Slow by design: a checker, not the traversal. */
+cqx_status cqx_trace_ref(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                         cqx_intersection* out, void* stream);
+
 /* Same with host buffers: copies in, runs, copies out, synchronizes. */
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                               uint32_t flags, float max_footprint, cqx_hit* hits);

@@ -10,16 +10,20 @@
 //               (the LDS node cache of variant 4)
 //   pair        persistent with two rays per lane (variant 5): both slots' child reads are taken
 //               before either slot steps on its read
-// Usage: core_emulation lane|persistent|top nodes.bin sub.bin rays.bin out.bin surf max_footprint
-//                       max_iters refill regroup
+//   traceref    traceRayRef per ray (trace_ref_core.h, the cqx_trace_ref kernel's walk); writes
+//               40-byte RayVolumeIntersection records
+// Usage: core_emulation lane|persistent|top|pair|traceref nodes.bin sub.bin rays.bin out.bin surf
+//                       max_footprint max_iters(traceref: root) refill regroup
 #define __device__
 #define __host__
 #define __forceinline__ inline
 #define __restrict__
 #include "traverse_core.h"
+#include "trace_ref_core.h"
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string_view>
 #include <vector>
 
@@ -66,6 +70,22 @@ int main(int argc, char** argv)
     const uint32_t top_nodes = uint32_t(internal < 512 ? internal : 512);
     const std::vector<uint32_t> top(P.nodes + kMaterialCount * 8, P.nodes + (kMaterialCount + top_nodes) * 8);
 
+    if (std::string_view(argv[1]) == "traceref") {
+        std::vector<cqx_intersection> res(n);
+        std::vector<uint32_t> sn(33);
+        std::vector<uint8_t> sx(33);
+        const uint32_t root = uint32_t(std::strtoul(argv[8], nullptr, 10)); // the max_iters slot carries the root
+        for (uint64_t i = 0; i < n; i++) {
+            cqx_ray r;
+            std::memcpy(&r, rays + 6 * i, sizeof(r));
+            res[i] = trace_ref_ray(P.nodes, root, r, sn.data(), sx.data(), 1);
+        }
+        FILE* f = std::fopen(argv[5], "wb");
+        if (!f) return 4;
+        const size_t w = std::fwrite(res.data(), sizeof(cqx_intersection), n, f);
+        std::fclose(f);
+        return w == n ? 0 : 5;
+    }
     if (pair) {
         // 64 lanes x 2 slots; slot j of lane l uses stack column 2 * l + j
         std::vector<uint32_t> stack2(128 * 33);

@@ -83,3 +83,22 @@ def test_core_deep_sparse_volumes(emu, tmp_path, cq, oracle, mode):
             got = _run(emu, tmp_path, cq, hv, rays, mode, surf, fp, 1 << 16, 16, 8)
             ref = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
             assert records_equal(got, ref).all(), f"{name} {mode} fp={fp}"
+
+
+def test_trace_ref_core_matches_oracle(emu, tmp_path, cq, oracle, vol128, rays128):
+    """traceRayRef's walk (trace_ref_core.h) on the host against the oracle's restatement, which
+    the compiled reference pins: hit, material and FP64 distance bit for bit."""
+    from tests.raysets import deep_volumes
+
+    cases = [(vol128, np.concatenate([rays128["camera"][::64], rays128["incoherent"][:2048], rays128["special"]]))]
+    cases += [(hv, rays[::4]) for hv, rays in deep_volumes(cq).values()]
+    for hv, rays in cases:
+        sub = cq.findSubDAGs(hv.nodes, hv.root)
+        paths = [str(tmp_path / f) for f in ("n.bin", "s.bin", "r.bin", "o.bin")]
+        np.ascontiguousarray(hv.nodes).tofile(paths[0])
+        np.ascontiguousarray(sub).tofile(paths[1])
+        np.ascontiguousarray(rays, dtype=np.float32).tofile(paths[2])
+        subprocess.run([emu, "traceref", *paths, "1", "-1.0", str(hv.root), "0", "0"], check=True, timeout=300)
+        got = np.fromfile(paths[3], dtype=cq.INTERSECTION_DTYPE)
+        ref = oracle.trace_ref(hv.nodes, hv.root, rays)
+        assert got.tobytes() == ref.tobytes()

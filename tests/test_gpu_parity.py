@@ -218,3 +218,28 @@ def test_gpu_host_entry_point(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
     g = gpu_ctx.intersect_host(dvol128, rays)
     o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays)
     assert records_equal(g, o).all()
+
+
+def test_gpu_trace_ref_matches_oracle(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
+    """cqx_trace_ref (the reference's brute-force raycast) against the oracle's restatement, byte
+    for byte, on the 128^3 ray sets and on volumes with 21-22-level SubDAGs."""
+    import torch
+
+    from tests.raysets import deep_volumes
+
+    cases = [(vol128, dvol128, np.concatenate([rays128["camera"][::16], rays128["incoherent"], rays128["special"]]))]
+    deep = []
+    for hv, rays in deep_volumes(cq).values():
+        dv = gpu_ctx.upload(hv)
+        deep.append(dv)
+        cases.append((hv, dv, rays))
+    for hv, dv, rays in cases:
+        r = torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda()
+        got = cq.traceRayRef(gpu_ctx, dv, r)
+        torch.cuda.synchronize()
+        ref = oracle.trace_ref(hv.nodes, hv.root, rays)
+        assert got.cpu().numpy().tobytes() == ref.tobytes()
+        hits = ref["hit"].sum()
+        assert 0 < hits < len(rays)
+    for dv in deep:
+        dv.free()
