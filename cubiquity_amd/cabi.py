@@ -106,6 +106,7 @@ SIGNATURES = {
     "cqx_hv_compact": (_ST, [_P]),
     "cqx_hv_checkpoint": (_ST, [_P]),
     "cqx_trace_ref": (_ST, [_P, _P, _P, _U64, _P, _P]),
+    "cqx_trace_ref_host": (_ST, [_P, _P, _P, _U64, _P]),
     "cqx_materials_load": (_ST, [ctypes.c_char_p, _P, _P, _P, _P]),
     "cqx_materials_save": (_ST, [ctypes.c_char_p, _P, _U32]),
     "cqx_materials_colour_table": (_ST, [_P, _U32, _P]),

@@ -64,6 +64,8 @@ namespace cqx {
 const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n);
 
 // Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).
+// Grows the context's staging buffer (host-buffer entry points) to at least `bytes`.
+cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes);
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                             const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
                             const cqx_launch_cfg* cfg, hipStream_t stream);

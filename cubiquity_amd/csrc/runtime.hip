@@ -342,6 +342,18 @@ cqx_status check_ctx(cqx_ctx* ctx)
 
 namespace cqx {
 
+cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes)
+{
+    if (bytes <= ctx->scratch_bytes) return CQX_OK;
+    HIP_TRY(hipDeviceSynchronize()); // launches in flight may use the old buffer
+    release(ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    if (hipMalloc(&ctx->scratch, bytes) != hipSuccess) return fail(CQX_ERR_OOM, "device staging of %zu bytes failed", bytes);
+    ctx->scratch_bytes = bytes;
+    return CQX_OK;
+}
+
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                             const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
                             const cqx_launch_cfg* cfg, hipStream_t stream)
@@ -685,15 +697,8 @@ cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ra
     if (st != CQX_OK) return st;
     if (n == 0) return CQX_OK;
     if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
-    const size_t need = size_t(n) * (sizeof(cqx_ray) + sizeof(cqx_hit));
-    if (need > ctx->scratch_bytes) {
-        HIP_TRY(hipDeviceSynchronize());
-        release(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
-        if (hipMalloc(&ctx->scratch, need) != hipSuccess) return fail(CQX_ERR_OOM, "device staging of %zu bytes failed", need);
-        ctx->scratch_bytes = need;
-    }
+    st = cqx::ensure_scratch(ctx, size_t(n) * (sizeof(cqx_ray) + sizeof(cqx_hit)));
+    if (st != CQX_OK) return st;
     cqx_ray* d_rays = static_cast<cqx_ray*>(ctx->scratch);
     cqx_hit* d_hits = reinterpret_cast<cqx_hit*>(static_cast<char*>(ctx->scratch) + size_t(n) * sizeof(cqx_ray));
     HIP_TRY(hipMemcpy(d_rays, rays, size_t(n) * sizeof(cqx_ray), hipMemcpyHostToDevice));

@@ -40,4 +40,22 @@ cqx_status cqx_trace_ref(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
     return CQX_OK;
 }
 
+cqx_status cqx_trace_ref_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                              cqx_intersection* out)
+{
+    if (!ctx || !vol) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    if (n == 0) return CQX_OK;
+    if (!rays || !out) return fail(CQX_ERR_INVALID_ARG, "null ray or result buffer");
+    HIP_TRY(hipSetDevice(ctx->device));
+    cqx_status st = cqx::ensure_scratch(ctx, size_t(n) * (sizeof(cqx_ray) + sizeof(cqx_intersection)));
+    if (st != CQX_OK) return st;
+    cqx_ray* d_rays = static_cast<cqx_ray*>(ctx->scratch);
+    auto* d_out = reinterpret_cast<cqx_intersection*>(static_cast<char*>(ctx->scratch) + size_t(n) * sizeof(cqx_ray));
+    HIP_TRY(hipMemcpy(d_rays, rays, size_t(n) * sizeof(cqx_ray), hipMemcpyHostToDevice));
+    st = cqx_trace_ref(ctx, vol, d_rays, n, d_out, nullptr);
+    if (st != CQX_OK) return st;
+    HIP_TRY(hipMemcpy(out, d_out, size_t(n) * sizeof(cqx_intersection), hipMemcpyDeviceToHost));
+    return CQX_OK;
+}
+
 } // extern "C"

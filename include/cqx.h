@@ -326,6 +326,9 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
  * them. rays and out are device pointers. Slow by design: a checker, not the traversal. */
 cqx_status cqx_trace_ref(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                          cqx_intersection* out, void* stream);
+/* Same with host buffers: copies in, runs, copies out, synchronizes. */
+cqx_status cqx_trace_ref_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                              cqx_intersection* out);
 
 /* Same with host buffers: copies in, runs, copies out, synchronizes. */
 cqx_status cqx_intersect_host(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,

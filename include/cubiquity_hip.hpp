@@ -5,8 +5,14 @@
 //   findSubDAGs(nodes, root)        raytracing.h:69    cqx_hip::findSubDAGs(nodes, count, root)
 //   intersectVolume(volume, subDAGs, ox..dz, surf, fp) cqx_hip::intersectVolume(gpuVolume, rays,
 //                                   raytracing.h:72-76     surf, fp) -> one record per ray
+//   traceRayRef(volume, ox..dz)     raytracing.h:78-81 cqx_hip::traceRayRef(gpuVolume, rays)
 //   RayVolumeIntersection           raytracing.h:48-55 cqx_intersection (same fields)
 //   NodeStore::rawBytesPtr/Count    storage.h:101-102  GpuVolume(ctx, rawBytes, byteCount, root)
+//   Volume (load, save, voxel,      storage.h:135-178  cqx_hip::HostVolume
+//     checkpoint, undo, redo, bake)
+//   fillBrush(volume, SphereBrush(c, r), material)     HostVolume::fillSphere / fillSpheres
+//                                   voxelization.h:91-126, voxelization.cpp:825-915
+//   Metadata::load (materials)      metadata.cpp:34-65 cqx_hip::loadMaterials
 //
 // Errors become cqx_hip::Error exceptions on this side of the ABI (the C ABI itself never throws).
 #pragma once
@@ -55,6 +61,50 @@ private:
     cqx_ctx* ctx_ = nullptr;
 };
 
+// The host-side volume: a node store and root with the reference Volume's operations.
+class HostVolume {
+public:
+    // Volume(filename) / Volume::load, storage.cpp:505-528
+    explicit HostVolume(const std::string& dag_path) { check(cqx_hv_load_dag(dag_path.c_str(), &hv_)); }
+    ~HostVolume() { cqx_hv_free(hv_); }
+    HostVolume(const HostVolume&) = delete;
+    HostVolume& operator=(const HostVolume&) = delete;
+
+    void save(const std::string& dag_path) const { check(cqx_hv_save_dag(hv_, dag_path.c_str())); }
+    uint32_t voxel(int32_t x, int32_t y, int32_t z) const
+    {
+        uint32_t m = 0;
+        check(cqx_hv_voxel(hv_, x, y, z, &m));
+        return m;
+    }
+    // fillBrush(volume, SphereBrush(centre, radius), material); returns the first appended node
+    uint64_t fillSphere(float cx, float cy, float cz, float radius, uint32_t material)
+    {
+        const cqx_sphere_edit e{ { cx, cy, cz }, radius, material };
+        return fillSpheres(&e, 1);
+    }
+    // Many spheres, applied in order (region-parallel on the host's cores)
+    uint64_t fillSpheres(const cqx_sphere_edit* edits, uint64_t n, int32_t threads = 0)
+    {
+        uint64_t first = 0;
+        check(cqx_hv_edit_spheres_mt(hv_, edits, n, threads, &first));
+        return first;
+    }
+    void checkpoint() { check(cqx_hv_checkpoint(hv_)); }
+    void undo() { check(cqx_hv_undo(hv_)); }
+    void redo() { check(cqx_hv_redo(hv_)); }
+    // Volume::bake's storage effect: unreachable nodes dropped, nodes renumbered
+    void bake() { check(cqx_hv_compact(hv_)); }
+
+    const uint32_t* nodes() const { return cqx_hv_nodes(hv_); }
+    uint64_t nodeCount() const { return cqx_hv_node_count(hv_); }
+    uint32_t root() const { return cqx_hv_root(hv_); }
+    cqx_host_volume* get() const { return hv_; }
+
+private:
+    cqx_host_volume* hv_ = nullptr;
+};
+
 // A node store resident in HBM. `raw_bytes` is the whole store including the 256 material nodes,
 // exactly what NodeStore::rawBytesPtr()/rawBytesCount() return (storage.h:101-102).
 class GpuVolume {
@@ -64,6 +114,13 @@ public:
     {
         if (byte_count % 32 != 0) throw Error(CQX_ERR_INVALID_ARG, "node store size is not a multiple of 32 bytes");
         check(cqx_volume_upload(ctx.get(), static_cast<const uint32_t*>(raw_bytes), byte_count / 32, root, sub, &vol_));
+    }
+    GpuVolume(Context& ctx, const HostVolume& hv) : GpuVolume(ctx, hv.nodes(), hv.nodeCount() * 32, hv.root()) {}
+    // After edits (first = the first appended node) or undo/redo (first = nodeCount()): copies the
+    // appended nodes and takes the new root and SubDAGs.
+    void update(const HostVolume& hv, uint64_t first)
+    {
+        check(cqx_volume_update(ctx_->get(), vol_, hv.nodes(), hv.nodeCount(), first, hv.root(), nullptr));
     }
     ~GpuVolume() { cqx_volume_free(ctx_->get(), vol_); }
     GpuVolume(const GpuVolume&) = delete;
@@ -97,6 +154,24 @@ inline void intersectVolumeDevice(const GpuVolume& vol, const cqx_ray* d_rays, u
 {
     check(cqx_intersect(vol.context().get(), vol.get(), d_rays, n, computeSurfaceProperties ? CQX_SURFACE_PROPERTIES : 0u,
                         maxFootprint, d_hits, nullptr, stream));
+}
+
+// traceRayRef (raytracing.cpp:480-535) for a batch, on the GPU: the reference's brute-force raycast.
+inline std::vector<cqx_intersection> traceRayRef(const GpuVolume& vol, const std::vector<cqx_ray>& rays)
+{
+    std::vector<cqx_intersection> out(rays.size());
+    check(cqx_trace_ref_host(vol.context().get(), vol.get(), rays.data(), rays.size(), out.data()));
+    return out;
+}
+
+// Materials of a volume's .toml metadata (Metadata::load, metadata.cpp:34-65), indexed by id.
+inline std::vector<cqx_material> loadMaterials(const std::string& toml_path)
+{
+    std::vector<cqx_material> m(256);
+    uint32_t count = 0;
+    check(cqx_materials_load(toml_path.c_str(), m.data(), &count, nullptr, nullptr));
+    m.resize(count);
+    return m;
 }
 
 // Reads a .dag file (Volume::load, storage.cpp:505-528) into a raw store + root.
