@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <mutex>
 #include <thread>
 
 namespace cqx {
@@ -147,6 +148,9 @@ public:
         }
     }
 
+    // Frees the hash index (the nodes stay): for when no more nodes will be interned.
+    void drop_index() { std::vector<uint32_t>().swap(slots_); }
+
 private:
     static constexpr uint32_t kEmpty = 0xffffffffu;
     std::vector<uint32_t> slots_;
@@ -186,41 +190,42 @@ public:
         for (size_t i = 0; i < all.size(); i++) all[i] = uint32_t(i);
 
         const int workers = int(std::max<int64_t>(1, std::min<int64_t>(threads_, ntasks)));
-        std::vector<NodeTable> local(workers);
-        std::vector<uint32_t> task_ref(ntasks), task_worker(ntasks);
+        // Each task is built into a worker-local table, then merged into the global table under a
+        // lock (children before parents, as created) and the local table starts afresh, so that
+        // memory stays near the size of the result. The merge order does not matter: the store is
+        // renumbered breadth-first from the root afterwards.
+        std::vector<uint32_t> task_ref(ntasks);
         std::atomic<int64_t> next{ 0 };
-        auto work = [&](int w) {
+        std::mutex merge_lock;
+        auto work = [&]() {
+            NodeTable local;
+            std::vector<uint32_t> remap;
             for (;;) {
                 const int64_t t = next.fetch_add(1);
                 if (t >= ntasks) break;
                 const int64_t x0 = (t % per_axis) * tside, y0 = ((t / per_axis) % per_axis) * tside,
                               z0 = (t / (per_axis * per_axis)) * tside;
-                task_ref[t] = build_region(local[w], x0, y0, z0, top, all);
-                task_worker[t] = uint32_t(w);
+                const uint32_t r = build_region(local, x0, y0, z0, top, all);
+                const std::vector<Node>& ln = local.nodes;
+                remap.assign(ln.size(), 0);
+                {
+                    std::lock_guard<std::mutex> g(merge_lock);
+                    for (size_t j = kMaterialCount; j < ln.size(); j++) {
+                        Node n = ln[j];
+                        for (int c = 0; c < 8; c++)
+                            if (n.c[c] >= kMaterialCount) n.c[c] = remap[n.c[c]];
+                        remap[j] = global.intern(n);
+                    }
+                }
+                task_ref[t] = r < kMaterialCount ? r : remap[r];
+                local = NodeTable();
             }
         };
         std::vector<std::thread> pool;
-        for (int w = 1; w < workers; w++) pool.emplace_back(work, w);
-        work(0);
+        for (int w = 1; w < workers; w++) pool.emplace_back(work);
+        work();
         for (auto& th : pool) th.join();
-
-        // Merge the per-worker tables (nodes appear after their children) into the global one.
-        std::vector<std::vector<uint32_t>> remap(workers);
-        for (int w = 0; w < workers; w++) {
-            const std::vector<Node>& ln = local[w].nodes;
-            remap[w].assign(ln.size(), 0);
-            for (size_t j = kMaterialCount; j < ln.size(); j++) {
-                Node n = ln[j];
-                for (int c = 0; c < 8; c++)
-                    if (n.c[c] >= kMaterialCount) n.c[c] = remap[w][n.c[c]];
-                remap[w][j] = global.intern(n);
-            }
-            std::vector<Node>().swap(local[w].nodes);
-        }
-        auto task_global = [&](int64_t t) {
-            uint32_t r = task_ref[t];
-            return r < kMaterialCount ? r : remap[task_worker[t]][r];
-        };
+        auto task_global = [&](int64_t t) { return task_ref[t]; };
         return build_top(global, 0, 0, 0, d_, top, per_axis, task_global);
     }
 
@@ -442,29 +447,27 @@ void canonical_order(std::vector<Node>& nodes, uint32_t& root)
         nodes.resize(kMaterialCount);
         return;
     }
+    // The output doubles as the breadth-first queue: out[i] holds the old children of the i-th
+    // node reached until its turn comes, then its children get their new indices.
     std::vector<uint32_t> remap(nodes.size(), 0xffffffffu);
-    std::vector<uint32_t> order;
-    order.reserve(nodes.size() - kMaterialCount);
+    std::vector<Node> out;
+    out.reserve(nodes.size());
+    out.insert(out.end(), nodes.begin(), nodes.begin() + kMaterialCount);
     remap[root] = kMaterialCount;
-    order.push_back(root);
-    for (size_t i = 0; i < order.size(); i++) {
-        const Node& n = nodes[order[i]];
+    out.push_back(nodes[root]);
+    for (size_t i = kMaterialCount; i < out.size(); i++) {
         for (int c = 0; c < 8; c++) {
-            const uint32_t ch = n.c[c];
-            if (ch >= kMaterialCount && remap[ch] == 0xffffffffu) {
-                remap[ch] = uint32_t(kMaterialCount + order.size());
-                order.push_back(ch);
+            const uint32_t ch = out[i].c[c];
+            if (ch < kMaterialCount) continue;
+            if (remap[ch] == 0xffffffffu) {
+                remap[ch] = uint32_t(out.size());
+                out.push_back(nodes[ch]);
             }
+            out[i].c[c] = remap[ch];
         }
     }
-    std::vector<Node> out(kMaterialCount + order.size());
-    std::copy(nodes.begin(), nodes.begin() + kMaterialCount, out.begin());
-    for (size_t i = 0; i < order.size(); i++) {
-        Node n = nodes[order[i]];
-        for (int c = 0; c < 8; c++)
-            if (n.c[c] >= kMaterialCount) n.c[c] = remap[n.c[c]];
-        out[kMaterialCount + i] = n;
-    }
+    std::vector<uint32_t>().swap(remap);
+    out.shrink_to_fit();
     nodes.swap(out);
     root = kMaterialCount;
 }
@@ -523,6 +526,7 @@ cqx_status cqx_hv_generate(const cqx_gen_params* p, int threads, cqx_host_volume
         r.c[7] = v;
         uint32_t root = g.intern(r);
         if (g.nodes.size() >= 0xffffffffull) return fail(CQX_ERR_LIMIT, "volume needs more than 2^32 nodes");
+        g.drop_index();
         canonical_order(g.nodes, root);
         cqx_host_volume* hv = new cqx_host_volume;
         hv->v.nodes = std::move(g.nodes);
