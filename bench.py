@@ -220,6 +220,57 @@ def bench_primary(cq, W, ctx, reps):
             "launch": {f: getattr(best, f) for f, _ in best._fields_}}
 
 
+def bench_large(cq, W, ctx, frames, w=None):
+    """The full-size edit workload: the 16384^3 volume with 1M spheres (built on the host), 64M
+    incoherent rays, then per frame 10k sphere edits (host, region-parallel), the append-only
+    device update and the traversal of all rays. `w` replaces the volume (same steps). The first
+    frame's edits include building the hash-consing index over the whole store."""
+    import torch
+
+    w = w or W.LARGE_16384
+    os.environ["CQX_PROGRESS"] = "1"
+    t0 = time.time()
+    hv = cq.HostVolume.generate(w.params())
+    build_s = time.time() - t0
+    log(f"large: {hv.node_count} nodes ({hv.node_count * 32 / 1e9:.2f} GB) built in {build_s:.1f}s")
+    rays = torch.from_numpy(cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)).cuda()
+    hits = torch.empty((rays.shape[0], 32), dtype=torch.uint8, device="cuda")
+    log("large: rays ready")
+    t0 = time.time()
+    dv = ctx.upload(hv)
+    upload_s = time.time() - t0
+    log(f"large: uploaded in {upload_s:.1f}s")
+    best, _ = cq.autotune(ctx, dv, rays, True)
+    trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
+    before = timed(trace, 2)
+    log(f"large: traversal {before:.2f} ms")
+    nodes0 = hv.node_count
+    edit_ms, update_ms, trace_ms, appended = [], [], [], []
+    for f in range(frames):
+        e = W.edit_frame_spheres(f, log2_size=w.log2_size, n_materials=255, n=W.LARGE_FRAME_EDITS)
+        t = time.perf_counter()
+        first = hv.edit_spheres(e)
+        edit_ms.append((time.perf_counter() - t) * 1e3)
+        t = time.perf_counter()
+        dv.update(hv, first)
+        update_ms.append((time.perf_counter() - t) * 1e3)
+        appended.append(hv.node_count - first)
+        trace_ms.append(timed(trace, 2))
+        log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms, update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms")
+    dv.free()
+    mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
+    n = rays.shape[0]
+    return {"volume": f"{w.side}^3 seed {w.seed}, {w.params().n_spheres} spheres, 255 materials", "nodes": nodes0, "store_gb": round(nodes0 * 32 / 1e9, 2),
+            "build_s": round(build_s, 1), "upload_s": round(upload_s, 2), "rays": n,
+            "trace_ms": round(before, 3), "mrays_per_s": round(n / (before * 1e-3) / 1e6, 1),
+            "edits_per_frame": W.LARGE_FRAME_EDITS, "host_edit_ms_per_frame": [round(x, 1) for x in edit_ms],
+            "host_edit_ms_steady": mean(edit_ms[1:]) if len(edit_ms) > 1 else mean(edit_ms),
+            "appended_nodes_per_frame": int(np.mean(appended)), "device_update_ms": mean(update_ms),
+            "trace_ms_after_edits": mean(trace_ms),
+            "frame_ms_steady": round(float(np.mean(edit_ms[1:] or edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 1),
+            "launch": {f2: getattr(best, f2) for f2, _ in best._fields_}}
+
+
 def bench_scale(cq, W, ctx, reps):
     """The traversal on larger volumes of the headline family (2^24 incoherent rays each, tuned
     launch): node store size, Mrays/s and algorithmic bandwidth, from MALL- to HBM-resident."""
@@ -296,6 +347,8 @@ def main():
     ap.add_argument("--no-pathtrace", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-edits", action="store_true")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the full-size edit workload (16384^3, 1M spheres; needs ~70 GB of host memory)")
     ap.add_argument("--scale", action="store_true",
                     help="also time the traversal on the 4096^3 and 8192^3 volumes of the same family")
     ap.add_argument("--log2-size", type=int, default=10,
@@ -430,6 +483,9 @@ def main():
     if not args.no_batched:
         out["batched"] = bench_batched(cq, W, ctx, max(2, args.steps // 4))
         log(f"batched: {out['batched']}")
+    if args.large:
+        out["large_16384"] = bench_large(cq, W, ctx, 3)
+        log(f"large: {out['large_16384']}")
     if args.scale:
         out["scale"] = bench_scale(cq, W, ctx, args.steps)
         log(f"scale: {out['scale']}")

@@ -71,6 +71,12 @@ EDIT_2048 = Workload("edit_2048", 11, 21, {"n_materials": 16}, None, 23, 1 << 24
 EDIT_FRAME_EDITS = 1000
 
 
+# The full-size edit workload: LARGE_16384 (16384^3, 1M spheres, 255 materials; ~800M nodes,
+# a ~26 GB store, whose host build needs ~70 GB), per frame 10k sphere edits seeded by frame, then
+# 64M incoherent rays.
+LARGE_FRAME_EDITS = 10000
+
+
 def edit_frame_spheres(frame, log2_size=11, n_materials=16, n=EDIT_FRAME_EDITS):
     """(n, 5) edits of one frame: centre U[0, N)^3, radius U[2, 16] * (N / 2048)^(1/3), material
     U{0..n_materials} (0 carves), numpy PCG64(7000 + frame)."""
