@@ -285,11 +285,12 @@ struct EditWalk {
                     if (b.hi[0] < fx0 || b.lo[0] > fx1 || b.hi[1] < fy0 || b.lo[1] > fy1 || b.hi[2] < fz0 ||
                         b.lo[2] > fz1)
                         continue;
-                    const bool all = b.contains(fx0, fy0, fz0) & b.contains(fx0, fy0, fz1) & b.contains(fx0, fy1, fz0) &
-                                     b.contains(fx0, fy1, fz1) & b.contains(fx1, fy0, fz0) & b.contains(fx1, fy0, fz1) &
-                                     b.contains(fx1, fy1, fz0) & b.contains(fx1, fy1, fz1);
                     const uint32_t child = out.c[id];
                     if (child == mat) continue;
+                    // all 8 corners inside (contains is a pure function: evaluation stops at the first outside)
+                    const bool all = b.contains(fx0, fy0, fz0) && b.contains(fx1, fy1, fz1) && b.contains(fx0, fy0, fz1) &&
+                                     b.contains(fx0, fy1, fz0) && b.contains(fx0, fy1, fz1) && b.contains(fx1, fy0, fz0) &&
+                                     b.contains(fx1, fy0, fz1) && b.contains(fx1, fy1, fz0);
                     uint32_t nc = child;
                     if (height >= 2 && !all) nc = fill(child, height - 1, x0, y0, z0);
                     else if (all) nc = mat;
