@@ -1,8 +1,9 @@
 // Launch autotuner for the traversal kernels.
 //
 // Tunes, on the caller's own rays and volume, the kernel variant (one ray per lane / persistent
-// with refill), rays per workgroup, persistent waves per CU, the regroup interval (traversal steps
-// between refill checks) and the refill threshold (idle lanes that trigger a refill). Each
+// with refill), the ray binning grid (none, or 4^3 to 16^3 cells), rays per workgroup, persistent
+// waves per CU, the regroup interval (traversal steps between refill checks) and the refill
+// threshold (idle lanes that trigger a refill). The binning prepass is inside the timed span. Each
 // candidate is timed with HIP events (median of 3 after a warm-up); the search is coordinate
 // descent from the built-in defaults, two sweeps. Results are cached in the context per
 // (SubDAG height, batch-size class = floor(log2 n)) and used by every launch whose configuration
@@ -120,8 +121,10 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
         return CQX_OK;
     };
     const uint32_t variants[] = { 2, 3, 5 };
+    const uint32_t bins[] = { 0, 2, 3, 4 };
     for (int pass = 0; pass < 2; pass++) {
         if ((st = sweep(&cqx_launch_cfg::variant, variants, 3)) != CQX_OK) return st;
+        if ((st = sweep(&cqx_launch_cfg::bin_bits, bins, 4)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::waves_per_cu, waves, 8)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::regroup_interval, regroups, 6)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::refill_lanes, refills, 5)) != CQX_OK) return st;

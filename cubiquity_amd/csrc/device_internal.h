@@ -34,6 +34,8 @@ struct cqx_ctx {
     uint64_t* d_counter = nullptr; // work counters of the persistent kernel (8 queues, 128 B apart)
     void* scratch = nullptr;       // staging for cqx_intersect_host
     size_t scratch_bytes = 0;
+    void* bins = nullptr; // ray binning: cell counts/offsets, then the order (u32) and cells (u16) of n rays
+    size_t bins_bytes = 0;
     hipStream_t capture_stream = nullptr; // private stream for graph capture
     cqx::PtPlan* pt_plan = nullptr;
     void* batch_desc = nullptr; // descriptor table of the batched launch

@@ -498,6 +498,7 @@ cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* p
 {
     cqx_launch_cfg c{};
     if (cfg) c = *cfg;
+    c.bin_bits = 0; // the bounce launches read their ray count on the device: no binning prepass
     PtPlan* p = ctx->pt_plan;
     if (p && p->vol == vol && p->width == pp->width && p->height == pp->height && p->spp == pp->spp &&
         p->bounces == pp->bounces && p->max_footprint == pp->max_footprint &&

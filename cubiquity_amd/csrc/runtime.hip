@@ -9,10 +9,14 @@
 //                           global counter with one atomic, handed out by ballot prefix
 //                           (variant 2); variant 3 also keeps the current node's 8 children in
 //                           registers
+//   k_bin_count / k_bin_scan / k_bin_scatter
+//                           optional prepass of the persistent variants: rays binned by origin
+//                           into a coarse Morton grid, handed out cell by cell
 // All keep the traversal stack in LDS (one column per lane, one row per node height).
 #include "device_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -83,10 +87,12 @@ struct SingleSource {
     TraceParams P;
     const cqx_ray* __restrict__ rays;
     cqx_hit* __restrict__ hits;
+    const uint32_t* __restrict__ order = nullptr; // binned launch: the ray behind each id
     __device__ __forceinline__ void fetch(uint64_t id, TraceParams&, const float*& ray, cqx_hit*& out) const
     {
-        ray = reinterpret_cast<const float*>(rays + id);
-        out = hits + id;
+        const uint64_t r = order ? uint64_t(order[id]) : id;
+        ray = reinterpret_cast<const float*>(rays + r);
+        out = hits + r;
     }
 };
 
@@ -311,6 +317,177 @@ __global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64
     }
 }
 
+// ---- ray binning -------------------------------------------------------------------------------
+// An incoherent batch taken in array order spreads the rays in flight on one XCD over the whole
+// volume, so its L2 holds few of the nodes they read. The prepass bins the rays by origin into a
+// coarse Morton-ordered grid over the volume's bounds; the persistent waves then take ray ids in
+// cell order through `order`, and as each XCD drains its own contiguous part of the id space, its
+// rays in flight come from a few neighbouring cells. The order within a cell is whatever the
+// atomics give: every ray's record is independent of it.
+constexpr uint32_t kMaxBinBits = 4;
+constexpr uint32_t kMaxCells = 1u << (3 * kMaxBinBits); // 4096
+constexpr uint32_t kBinTile = 16384;                    // rays per scatter tile
+
+struct BinGrid {
+    float lo[3];
+    float scale[3]; // cells per unit length along each axis
+    uint32_t bits;  // 2^bits cells per axis
+};
+
+__device__ __forceinline__ uint32_t bin_cell(const BinGrid& g, const float* r)
+{
+    const uint32_t side = 1u << g.bits;
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float f = (r[a] - g.lo[a]) * g.scale[a];
+        // clamped to [0, side): origins outside the bounds go to the border cells, NaN to cell 0
+        const uint32_t k = f >= 0.0f ? (f < float(side) ? uint32_t(f) : side - 1u) : 0u;
+#pragma unroll
+        for (uint32_t b = 0; b < kMaxBinBits; b++) c |= ((k >> b) & 1u) << (3 * b + a);
+    }
+    return c; // < 2^(3 bits)
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t count)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Cell of every ray, and the number of rays per cell (counts must start at zero).
+__global__ void __launch_bounds__(256) k_bin_count(const cqx_ray* __restrict__ rays, uint64_t n, BinGrid g,
+                                                   uint32_t ncells, uint16_t* __restrict__ cells,
+                                                   uint32_t* __restrict__ counts)
+{
+    __shared__ uint32_t hist[kMaxCells];
+    for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x) hist[b] = 0u;
+    __syncthreads();
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t c = bin_cell(g, reinterpret_cast<const float*>(rays + i));
+        cells[i] = uint16_t(c);
+        atomicAdd(&hist[c], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x)
+        if (hist[b]) atomicAdd(&counts[b], hist[b]);
+}
+
+// Exclusive prefix sum of counts[0, ncells) in place (ncells <= 4096: four per thread).
+__global__ void __launch_bounds__(1024) k_bin_scan(uint32_t* __restrict__ counts, uint32_t ncells)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        const uint32_t i = 4 * t + k;
+        v[k] = i < ncells ? counts[i] : 0u;
+        sum += v[k];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t x = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+        const uint32_t i = 4 * t + k;
+        if (i < ncells) counts[i] = run;
+        run += v[k];
+    }
+}
+
+// order[offsets[c] ...] = the ids of the rays in cell c. Per tile of kBinTile rays: a histogram in
+// LDS, one global reservation per cell present, then each ray takes the next slot of its cell.
+// The reservations of all tiles for cell c add up to its count, so every slot of [0, n) is
+// written once.
+__global__ void __launch_bounds__(256) k_bin_scatter(const uint16_t* __restrict__ cells, uint64_t n, uint32_t ncells,
+                                                     uint32_t* __restrict__ offsets, uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t cur[kMaxCells];
+    const uint64_t tiles = (n + kBinTile - 1) / kBinTile;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint64_t first = tile * kBinTile;
+        const uint64_t last = first + kBinTile < n ? first + kBinTile : n;
+        for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x) cur[b] = 0u;
+        __syncthreads();
+        for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x) atomicAdd(&cur[cells[i]], 1u);
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x)
+            if (cur[b]) cur[b] = atomicAdd(&offsets[b], cur[b]);
+        __syncthreads();
+        for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x)
+            order[atomicAdd(&cur[cells[i]], 1u)] = uint32_t(i);
+        __syncthreads();
+    }
+}
+
+// The binning grid: 2^bits cells per axis over the union of the boxes of the non-empty SubDAGs.
+// False when every SubDAG is empty (nothing to bin for).
+bool bin_grid(const cqx_dvolume* vol, uint32_t bits, BinGrid& g)
+{
+    double lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+    bool any = false;
+    for (const cqx_subdag& s : vol->hsub) {
+        if (s.node == 0) continue;
+        const double side = std::ldexp(1.0, s.height);
+        for (int a = 0; a < 3; a++) {
+            const double l = s.lower[a], h = double(s.lower[a]) + side;
+            lo[a] = any ? std::min(lo[a], l) : l;
+            hi[a] = any ? std::max(hi[a], h) : h;
+        }
+        any = true;
+    }
+    if (!any) return false;
+    g.bits = bits;
+    for (int a = 0; a < 3; a++) {
+        g.lo[a] = float(lo[a]);
+        g.scale[a] = float(std::ldexp(1.0, int(bits)) / (hi[a] - lo[a]));
+    }
+    return true;
+}
+
+// Binning buffers of the context: kMaxCells counts, then n order entries (u32) and n cells (u16).
+cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
+{
+    const size_t bytes = size_t(kMaxCells) * 4 + size_t(n) * 4 + size_t(n) * 2;
+    if (bytes <= ctx->bins_bytes) return CQX_OK;
+    HIP_TRY(hipDeviceSynchronize()); // launches in flight may use the old buffer
+    release(ctx->bins);
+    ctx->bins = nullptr;
+    ctx->bins_bytes = 0;
+    if (hipMalloc(&ctx->bins, bytes) != hipSuccess) return fail(CQX_ERR_OOM, "ray binning buffer of %zu bytes failed", bytes);
+    ctx->bins_bytes = bytes;
+    return CQX_OK;
+}
+
+// Runs the binning prepass of n rays on `stream`; `order` receives the device pointer of the
+// resulting id -> ray map.
+cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid& g, hipStream_t stream,
+                    const uint32_t*& order)
+{
+    cqx_status st = ensure_bins(ctx, n);
+    if (st != CQX_OK) return st;
+    uint32_t* counts = static_cast<uint32_t*>(ctx->bins);
+    uint32_t* ord = counts + kMaxCells;
+    uint16_t* cells = reinterpret_cast<uint16_t*>(ord + n);
+    const uint32_t ncells = 1u << (3 * g.bits);
+    const uint64_t grid_cap = uint64_t(ctx->cu_count) * 4;
+    const uint32_t count_blocks = uint32_t(std::max<uint64_t>(1, std::min(grid_cap, (n + 255) / 256)));
+    const uint32_t scatter_blocks = uint32_t(std::max<uint64_t>(1, std::min(grid_cap, (n + kBinTile - 1) / kBinTile)));
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, counts, ncells);
+    hipLaunchKernelGGL(k_bin_count, dim3(count_blocks), dim3(256), 0, stream, rays, n, g, ncells, cells, counts);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, stream, counts, ncells);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(scatter_blocks), dim3(256), 0, stream, cells, n, ncells, counts, ord);
+    HIP_TRY(hipGetLastError());
+    order = ord;
+    return CQX_OK;
+}
+
 // Launch shape of the persistent kernel: CUs x waves_per_cu waves, at most one lane per ray.
 // `chunk`: ray ids a wave claims per atomic, about a quarter of a wave's share (64 to 4096).
 cqx_status persistent_shape(const cqx_ctx* ctx, const cqx_launch_cfg* cfg, uint32_t block, uint64_t n,
@@ -375,6 +552,19 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
     if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
 
     const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
+    const uint32_t bin_bits = cfg ? cfg->bin_bits : 0u;
+    const uint32_t* order = nullptr;
+    if (bin_bits) {
+        if (bin_bits > kMaxBinBits) return fail(CQX_ERR_INVALID_ARG, "bin_bits %u outside 0..%u", bin_bits, kMaxBinBits);
+        if (variant < 2 || variant > 5) return fail(CQX_ERR_INVALID_ARG, "ray binning needs a persistent variant (2-5)");
+        if (n_dev) return fail(CQX_ERR_INVALID_ARG, "ray binning needs the ray count on the host");
+        if (n > 0xffffffffull) return fail(CQX_ERR_LIMIT, "ray binning takes fewer than 2^32 rays");
+        BinGrid g;
+        if (bin_grid(vol, bin_bits, g)) {
+            const cqx_status bst = bin_rays(ctx, rays, n, g, stream, order);
+            if (bst != CQX_OK) return bst;
+        }
+    }
     if (variant == 1) {
         const uint64_t blocks = (n + block - 1) / block;
         if (blocks > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "too many rays for one launch");
@@ -386,7 +576,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         const size_t lds2 = 2 * lds;
         if (lds2 > size_t(ctx->lds_per_block))
             return fail(CQX_ERR_LIMIT, "two stacks per lane need %zu B of LDS per block", lds2);
-        SingleSource src{ P, rays, hits };
+        SingleSource src{ P, rays, hits, order };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter));
         hipLaunchKernelGGL(k_intersect_pair, dim3(blocks), dim3(block), lds2, stream, src, n, n_dev,
@@ -404,7 +594,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
             if (lds_total > size_t(ctx->lds_per_block))
                 return fail(CQX_ERR_LIMIT, "stack and %u LDS nodes need %zu B of LDS per block", top_nodes, lds_total);
         }
-        SingleSource src{ P, rays, hits };
+        SingleSource src{ P, rays, hits, order };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
                            reinterpret_cast<unsigned long long*>(ctx->d_counter));
         unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
@@ -464,6 +654,7 @@ cqx_status cqx_destroy(cqx_ctx* ctx)
     cqx::destroy_pt_plan(ctx->pt_plan);
     release(ctx->d_counter);
     release(ctx->scratch);
+    release(ctx->bins);
     release(ctx->batch_desc);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     delete ctx;
@@ -636,6 +827,7 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     if (st != CQX_OK) return st;
     if (n_batches == 0) return CQX_OK;
     if (!batches) return fail(CQX_ERR_INVALID_ARG, "null batch array");
+    if (cfg && cfg->bin_bits) return fail(CQX_ERR_INVALID_ARG, "ray binning applies to cqx_intersect only");
     std::vector<BatchDesc> desc(n_batches);
     uint64_t total = 0;
     int max_height = 0;

@@ -283,6 +283,12 @@ typedef struct {
     uint32_t max_iters;        /* per-ray iteration cap (0 = built-in 1<<22) */
     uint32_t lds_nodes;        /* variant 4: internal nodes [256, 256 + lds_nodes) read from a copy in
                                   LDS (the top of a breadth-first store; 0 = 512) */
+    uint32_t bin_bits;         /* variants 2-5, cqx_intersect only: 0 = rays are taken in array order;
+                                  1..4 = a prepass first bins the rays by origin into 2^(3 bin_bits)
+                                  Morton-ordered cells over the volume's SubDAG bounds, and the
+                                  persistent waves take them cell by cell (each XCD a contiguous run
+                                  of cells), so that rays in flight together read nearby nodes.
+                                  Results are the same, in the caller's order. n < 2^32. */
 } cqx_launch_cfg;
 
 /* intersectVolume over n rays, batched: rays and hits are DEVICE pointers; stream is a

@@ -51,6 +51,43 @@ def test_gpu_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, rays128, s
     assert eq.all(), _report(set_name, eq, rays, g, o)
 
 
+@pytest.mark.parametrize("variant", [2, 3, 4, 5])
+@pytest.mark.parametrize("bin_bits", [1, 3, 4])
+@pytest.mark.parametrize("set_name", ["camera", "incoherent", "special"])
+def test_gpu_binned_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, rays128, set_name, bin_bits, variant):
+    """Rays binned by origin before the persistent launch: the same records, in the caller's order
+    (the special set has origins outside the volume, infinite and NaN, which bin to border cells)."""
+    rays = rays128[set_name]
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    o = oracle.intersect(vol128.nodes, sub, rays, True, -1.0, threads=8)
+    g = _gpu(gpu_ctx, dvol128, rays, True, -1.0, cq.LaunchCfg(variant=variant, bin_bits=bin_bits))
+    eq = records_equal(g, o)
+    assert eq.all(), _report(set_name + " binned", eq, rays, g, o)
+
+
+def test_gpu_binned_ragged_counts(cq, gpu_ctx, dvol128, vol128):
+    """Binned launches of 1 ray, a few rays, and counts either side of the 16384-ray scatter tile,
+    against unbinned launches; then a smaller batch after a larger one (stale order entries)."""
+    rays = cq.incoherent_rays(vol128, 7, 11, 40000)
+    full = _gpu(gpu_ctx, dvol128, rays, True, -1.0, cq.LaunchCfg(variant=2))
+    for n in (40000, 1, 17, 16383, 16384, 16385, 32769):
+        g = _gpu(gpu_ctx, dvol128, rays[:n], True, -1.0, cq.LaunchCfg(variant=2, bin_bits=3))
+        assert records_equal(g, full[:n]).all(), n
+
+
+def test_gpu_binning_rejects_bad_configs(cq, gpu_ctx, dvol128, rays128):
+    import torch
+
+    r = torch.from_numpy(rays128["incoherent"][:64]).cuda()
+    for cfg in (cq.LaunchCfg(variant=2, bin_bits=5), cq.LaunchCfg(variant=1, bin_bits=2)):
+        with pytest.raises(cq.CqxError):
+            cq.intersectVolume(gpu_ctx, dvol128, r, True, cfg=cfg)
+    h = torch.zeros((64, 32), dtype=torch.uint8, device="cuda")
+    with pytest.raises(cq.CqxError):
+        cq.intersect_batched(gpu_ctx, [(dvol128, r, h, True, -1.0)], cfg=cq.LaunchCfg(bin_bits=3))
+    torch.cuda.synchronize()
+
+
 def test_gpu_matches_golden(cq, gpu_ctx, dvol128, vol128):
     """Against the compiled reference's own outputs stored as fixtures (no reference needed)."""
     z = np.load(GOLDEN, allow_pickle=False)
@@ -149,7 +186,11 @@ def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     g3 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=3))
     g4 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=4, lds_nodes=1024))
     g5 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=5))
+    g2b = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=2, bin_bits=3))
+    g3b = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=3, bin_bits=4))
     print(f"gpu {time.time() - t0:.2f}s", flush=True)
+    assert records_equal(g2b, g2).all()
+    assert records_equal(g3b, g2).all()
     assert records_equal(g1, g2).all()
     assert records_equal(g3, g2).all()
     assert records_equal(g4, g2).all()
@@ -178,7 +219,7 @@ def test_gpu_autotuned_launch_is_exact(cq, oracle, gpu_ctx, vol128, rays128):
     rays = rays128["incoherent"]
     r = torch.from_numpy(rays).cuda()
     best, ms = cq.autotune(ctx, dv, r, True, -1.0)
-    assert best.variant in (1, 2, 3, 5) and ms > 0
+    assert best.variant in (1, 2, 3, 5) and best.bin_bits in (0, 2, 3, 4) and ms > 0
     auto = cq.intersectVolume(ctx, dv, r, True).cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
     o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True)
     assert records_equal(auto, o).all()
