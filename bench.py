@@ -125,6 +125,16 @@ def bench_pathtrace(cq, W, ctx, reps):
     best = min(out.values())
     out["paths_per_frame"] = 1280 * 720 * 16
     out["mpaths_per_s"] = round(1280 * 720 * 16 / (best * 1e-3) / 1e6, 2)
+    # the reference CPU tracer's shading (traceSingleRay: sun and sky shadow rays, one reflection,
+    # LOD 0.0035) for one 1280x720 frame of the same camera
+    import torch
+
+    rays = torch.from_numpy(cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, 1280, 720)).cuda()
+    colours = np.zeros((256, 3), np.float32)
+    colours[1:10] = alb
+    img = torch.empty((rays.shape[0], 3), dtype=torch.float32, device="cuda")
+    out["reference_shading_frame_ms"] = round(timed(lambda: cq.demo_trace(ctx, dv, rays, colours, seed=1, out=img),
+                                                    reps), 3)
     dv.free()
     return out
 

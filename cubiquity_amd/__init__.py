@@ -21,7 +21,7 @@ __all__ = [
     "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
     "incoherent_rays", "gen_default_params", "hits_to_intersections", "CqxError", "GenParams",
     "LaunchCfg", "HIT_DTYPE", "INTERSECTION_DTYPE", "SUBDAG_DTYPE", "CQX_SURFACE_PROPERTIES",
-    "MAX_FOOTPRINT_DISABLED", "decode_info",
+    "MAX_FOOTPRINT_DISABLED", "decode_info", "demo_trace",
 ]
 
 
@@ -447,6 +447,37 @@ def pathtrace(ctx, dvol, setup, mode=0, cfg=None, out=None, stream=None):
     cfgp = ctypes.byref(cfg) if cfg is not None else None
     check(lib().cqx_pathtrace(ctx._h, dvol._h, ctypes.byref(setup.params), int(mode), cfgp,
                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
+    return out
+
+
+def demo_trace(ctx, dvol, rays, colours, max_footprint=0.0035, gamma=0.0, add_noise=True, include_sun=True,
+               include_sky=True, seed=0, out=None, accumulate=False, cfg=None, stream=None):
+    """The reference CPU tracer's shading (cqx_demo_trace, PathtracingDemo::traceSingleRay,
+    src/application/commands/view/pathtracing_demo.cpp:150-194) for one primary ray per pixel.
+
+    ``rays``: float32 GPU tensor (n, 6); ``colours``: (256, 3) material colours (e.g.
+    ``Materials.colour_table()``). Returns (or fills / adds to, with ``accumulate``) a float32 GPU
+    tensor (n, 3). Per-pixel generator states from ``seed``; gamma 0 is the reference's 1/2.2."""
+    import torch
+
+    if not (isinstance(rays, torch.Tensor) and rays.is_cuda and rays.dtype == torch.float32):
+        raise TypeError("rays must be a float32 GPU tensor of shape (n, 6)")
+    rays = rays.contiguous().view(-1, 6)
+    n = rays.shape[0]
+    col = np.ascontiguousarray(colours, dtype=np.float32).reshape(256, 3)
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((n, 3), dtype=torch.float32, device=rays.device)
+    if out.numel() < n * 3 or out.dtype != torch.float32:
+        raise ValueError("out must be a float32 tensor of n x 3")
+    if stream is None:
+        stream = torch.cuda.current_stream(rays.device).cuda_stream
+    prm = cabi.DemoParams(col.ctypes.data, float(max_footprint), float(gamma), int(bool(add_noise)),
+                          int(bool(include_sun)), int(bool(include_sky)), int(bool(accumulate)), int(seed))
+    cfgp = ctypes.byref(cfg) if cfg is not None else None
+    check(lib().cqx_demo_trace(ctx._h, dvol._h, ctypes.c_void_p(rays.data_ptr()), n, ctypes.byref(prm), cfgp,
+                               ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
     return out
 
 

@@ -57,6 +57,15 @@ class PtParams(ctypes.Structure):
 assert ctypes.sizeof(PtParams) == 128
 
 
+class DemoParams(ctypes.Structure):
+    _fields_ = [("colours", ctypes.c_void_p), ("max_footprint", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("add_noise", ctypes.c_uint32), ("include_sun", ctypes.c_uint32), ("include_sky", ctypes.c_uint32),
+                ("accumulate", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
+assert ctypes.sizeof(DemoParams) == 40
+
+
 class SphereEdit(ctypes.Structure):
     _fields_ = [("centre", ctypes.c_float * 3), ("radius", ctypes.c_float), ("material", ctypes.c_uint32)]
 
@@ -127,6 +136,7 @@ SIGNATURES = {
     "cqx_autotune": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P, _P]),
     "cqx_autotune_clear": (_ST, [_P]),
     "cqx_pathtrace": (_ST, [_P, _P, _P, _U32, _P, _P, _P]),
+    "cqx_demo_trace": (_ST, [_P, _P, _P, _U64, _P, _P, _P, _P]),
     "cqx_pathtrace_adjoint": (_ST, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 

@@ -1,0 +1,508 @@
+// Reference-shading mode: the reference CPU path tracer's per-pixel shading,
+// PathtracingDemo::traceSingleRay (src/application/commands/view/pathtracing_demo.cpp:150-194),
+// for a batch of primary rays on the gfx950 traversal.
+//
+// Per pixel p with primary ray r (all FP32 unless noted, operations in the reference's order):
+//   i0 = intersectVolume(r, surface properties, maxFootprint)                         :153-156
+//   miss: colour (0.8, 0.8, 1.0)                                                      :151
+//   hit:  surf0 = colours[m0] x noise(pos0)                         surfaceColour, :44-59
+//         direct0 = gather(pos0, n0)                                gatherLighting, :82-117
+//         reflected ray from pos0 + n0 x 0.01 along normalize(n0 + randomPointInUnitSphere())
+//         i1 = intersectVolume(reflected, surface properties, maxFootprint)
+//         indirect0 = i1 hit ? surf1 x gather(pos1, n1) : 0
+//         colour = pow(surf0 x (direct0 + indirect0), 1/2.2)                           :187-190
+//   gather(pos, n): shadow rays (no surface properties) from pos + n x 0.001:
+//         sun: along normalize(1, -2, 10); if unoccluded += 0.1 x max(dot(sunDir, n), 0)
+//         sky: along normalize(n + randomPointInUnitSphere()); if unoccluded += 1.5
+//   noise(pos) = float(0.9 + 0.1 x ((fnv1a(int(pos + 0.499)) & 0xff) / 255)) (the mapping in
+//         double: the reference's literals are doubles)               positionBasedNoise, :34-42
+//   randomPointInUnitSphere: s = uint32(bit_mix(s)); the three 10-bit fields of s mapped by
+//         (v - 511.5) / 511.5, repeated while the squared length is >= 1                   :61-80
+// The reference draws from ONE generator state shared by all pixels in scan order, so a pixel's
+// draws depend on how many draws every earlier pixel made (which depends on their hits). Here each
+// pixel has its own state, uint32(bit_mix(seed x 2^32 + p + 1)) (17, the reference's initial
+// state, if that is 0); the draw function, and the draw order within a pixel (sky at i0,
+// reflection, sky at i1), are the reference's. A draw gives up after 1024 generator steps with
+// (0, 0, 0): the reference loops forever on a state cycle that never lands inside the sphere.
+// Rays the traversal flags as anomalies (the reference would not return) count as misses.
+//
+// Wavefront: trace primaries -> k_demo_primary (sun/sky shadow rays and the reflected ray, each
+// list compacted by one atomic per wave) -> trace shadow rays -> k_demo_visibility -> trace
+// reflected rays -> k_demo_bounce (second-hit shadow rays) -> trace -> k_demo_visibility ->
+// k_demo_resolve. Ray counts stay on the device; the persistent traversal kernel reads them.
+// Visibility is kept per (pixel, shadow ray) and summed in the reference's order at the end.
+#include "device_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+using namespace cqxd;
+using cqx::fail;
+using cqx::release;
+
+namespace {
+
+constexpr int kMaxDraws = 1024;
+// shadow-ray kinds: a shadow ray's tag is pixel x 4 + kind, also its visibility byte
+constexpr uint32_t kSun0 = 0, kSky0 = 1, kSun1 = 2, kSky1 = 3;
+
+struct DemoDev {
+    const float* colours; // 256 x 3, device
+    float sun[3];
+    float gamma;
+    uint32_t add_noise, include_sun, include_sky, accumulate;
+    uint64_t seed;
+};
+
+// Internals::bit_mix, src/library/base.cpp:72-77 (the 'xmxmx' mixer)
+__device__ __forceinline__ uint64_t bit_mix(uint64_t b)
+{
+    b = ((b >> 32) ^ b) * 0x0e9846af9b1a615dull;
+    b = ((b >> 32) ^ b) * 0x0e9846af9b1a615dull;
+    return (b >> 28) ^ b;
+}
+
+// Internals::fnv1a (src/library/base.cpp:62-69) over the 12 bytes of an ivec3
+__device__ __forceinline__ uint64_t fnv1a_ivec3(const int32_t (&q)[3])
+{
+    uint64_t h = 0xcbf29ce484222325ull;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const uint32_t w = uint32_t(q[a]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            h ^= (w >> (8 * k)) & 0xffu;
+            h *= 0x00000100000001B3ull;
+        }
+    }
+    return h;
+}
+
+__device__ __forceinline__ float position_noise(const float (&pos)[3])
+{
+    int32_t q[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) q[a] = int32_t(pos[a] + 0.499f);
+    const uint32_t h = uint32_t(fnv1a_ivec3(q));
+    const float noise = float(h & 0xffu) / 255.0f;
+    return float((double(noise) * 0.1) + 0.9);
+}
+
+__device__ __forceinline__ void random_point(uint32_t& s, float (&r)[3])
+{
+    for (int k = 0; k < kMaxDraws; k++) {
+        s = uint32_t(bit_mix(s));
+        r[0] = float(s & 0x3FFu);
+        r[1] = float((s >> 10) & 0x3FFu);
+        r[2] = float((s >> 20) & 0x3FFu);
+#pragma unroll
+        for (int a = 0; a < 3; a++) r[a] = (r[a] - 511.5f) / 511.5f;
+        float d = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) d = d + r[a] * r[a];
+        if (!(d >= 1.0f)) return;
+    }
+    r[0] = r[1] = r[2] = 0.0f;
+}
+
+// linalg normalize: a / sqrt(dot(a, a)), dot a left fold from 0
+__device__ __forceinline__ void normalize3(const float (&v)[3], float (&out)[3])
+{
+    float d = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) d = d + v[a] * v[a];
+    const float len = sqrtf(d);
+#pragma unroll
+    for (int a = 0; a < 3; a++) out[a] = v[a] / len;
+}
+
+__device__ __forceinline__ uint32_t pixel_state(uint64_t seed, uint64_t p)
+{
+    const uint32_t s = uint32_t(bit_mix((seed << 32) + p + 1u));
+    return s ? s : 17u;
+}
+
+// Face normal (components -1, -0, +0, +1) from a hit record's normal code.
+__device__ __forceinline__ void face_normal(uint32_t info, float (&n)[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const uint32_t code = (info >> (9 + 2 * a)) & 3u;
+        const float mag = (code & 1u) ? 1.0f : 0.0f;
+        n[a] = (code & 2u) ? -mag : mag;
+    }
+}
+
+// Hit position o + d t (intersectVolume, raytracing.cpp:464) and surfaceColour.
+__device__ __forceinline__ void hit_surface(const DemoDev& d, const cqx_ray& r, const cqx_hit& h, float (&pos)[3],
+                                            float (&nrm)[3], float (&col)[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++) pos[a] = r.o[a] + r.d[a] * h.t;
+    face_normal(h.info, nrm);
+    const uint32_t m = (h.info >> 16) & 0xffu;
+#pragma unroll
+    for (int a = 0; a < 3; a++) col[a] = d.colours[m * 3 + a];
+    if (d.add_noise) {
+        const float noise = position_noise(pos);
+#pragma unroll
+        for (int a = 0; a < 3; a++) col[a] = col[a] * noise;
+    }
+}
+
+// The shadow rays of gatherLighting at (pos, n); the sky direction takes one draw from s.
+__device__ __forceinline__ void shadow_rays(const DemoDev& d, const float (&pos)[3], const float (&n)[3], uint32_t& s,
+                                            cqx_ray& sun, cqx_ray& sky)
+{
+    float so[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float offset = n[a] * 0.001f;
+        so[a] = pos[a] + offset;
+        sun.o[a] = so[a];
+        sun.d[a] = d.sun[a];
+        sky.o[a] = so[a];
+    }
+    if (d.include_sky) {
+        float pt[3], v[3], dir[3];
+        random_point(s, pt);
+#pragma unroll
+        for (int a = 0; a < 3; a++) v[a] = n[a] + pt[a];
+        normalize3(v, dir);
+#pragma unroll
+        for (int a = 0; a < 3; a++) sky.d[a] = dir[a];
+    }
+}
+
+// Slot of this lane's entry in a list compacted by one atomic per wave (call with the whole wave).
+__device__ __forceinline__ uint32_t wave_slot(bool take, uint32_t* count)
+{
+    const uint64_t mask = __ballot(take);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t base = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if (int(lane) == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader);
+    }
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__global__ void k_demo_zero(uint32_t* __restrict__ counts)
+{
+    if (threadIdx.x < 4) counts[threadIdx.x] = 0u;
+}
+
+// First hits: surface colour, the two shadow rays and the reflected ray; clears the pixel's
+// visibility bytes.
+__global__ void __launch_bounds__(256) k_demo_primary(DemoDev d, uint64_t n, const cqx_ray* __restrict__ rays,
+                                                      const cqx_hit* __restrict__ hits, uint32_t* __restrict__ state,
+                                                      uint32_t* __restrict__ info0, float* __restrict__ surf0,
+                                                      uint32_t* __restrict__ vis, cqx_ray* __restrict__ sh_rays,
+                                                      uint32_t* __restrict__ sh_tags, uint32_t* __restrict__ sh_count,
+                                                      cqx_ray* __restrict__ rf_rays, uint32_t* __restrict__ rf_tags,
+                                                      uint32_t* __restrict__ rf_count)
+{
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool hit = false;
+    cqx_ray sun, sky, refl;
+    if (p < n) {
+        const cqx_hit h = hits[p];
+        info0[p] = h.info;
+        vis[p] = 0u;
+        hit = (h.info & CQX_HIT_FLAG) != 0;
+        if (hit) {
+            float pos[3], nrm[3], col[3];
+            hit_surface(d, rays[p], h, pos, nrm, col);
+#pragma unroll
+            for (int a = 0; a < 3; a++) surf0[p * 3 + a] = col[a];
+            uint32_t s = pixel_state(d.seed, p);
+            shadow_rays(d, pos, nrm, s, sun, sky);
+            float pt[3], v[3], dir[3];
+            random_point(s, pt);
+#pragma unroll
+            for (int a = 0; a < 3; a++) v[a] = nrm[a] + pt[a];
+            normalize3(v, dir);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                refl.o[a] = pos[a] + nrm[a] * 0.01f;
+                refl.d[a] = dir[a];
+            }
+            state[p] = s;
+        }
+    }
+    const bool take_sun = hit && d.include_sun, take_sky = hit && d.include_sky;
+    uint32_t k = wave_slot(take_sun, sh_count);
+    if (take_sun) {
+        sh_rays[k] = sun;
+        sh_tags[k] = uint32_t(p) * 4u + kSun0;
+    }
+    k = wave_slot(take_sky, sh_count);
+    if (take_sky) {
+        sh_rays[k] = sky;
+        sh_tags[k] = uint32_t(p) * 4u + kSky0;
+    }
+    k = wave_slot(hit, rf_count);
+    if (hit) {
+        rf_rays[k] = refl;
+        rf_tags[k] = uint32_t(p);
+    }
+}
+
+// A shadow ray that hits nothing lets its light through.
+__global__ void __launch_bounds__(256) k_demo_visibility(const uint32_t* __restrict__ count, uint64_t cap,
+                                                         const uint32_t* __restrict__ tags,
+                                                         const cqx_hit* __restrict__ hits, uint8_t* __restrict__ vis)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t c = *count;
+    if (i >= c || i >= cap) return;
+    vis[tags[i]] = (hits[i].info & CQX_HIT_FLAG) ? 0u : 1u;
+}
+
+// Second hits (the reflected rays): surface colour and the second gatherLighting's shadow rays.
+__global__ void __launch_bounds__(256) k_demo_bounce(DemoDev d, uint64_t cap, const uint32_t* __restrict__ rf_count,
+                                                     const cqx_ray* __restrict__ rf_rays,
+                                                     const uint32_t* __restrict__ rf_tags,
+                                                     const cqx_hit* __restrict__ rf_hits, const uint32_t* __restrict__ state,
+                                                     uint32_t* __restrict__ info1, float* __restrict__ surf1,
+                                                     cqx_ray* __restrict__ sh_rays, uint32_t* __restrict__ sh_tags,
+                                                     uint32_t* __restrict__ sh_count)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t c = *rf_count;
+    bool hit = false;
+    uint32_t p = 0;
+    cqx_ray sun, sky;
+    if (i < c && i < cap) {
+        p = rf_tags[i];
+        const cqx_hit h = rf_hits[i];
+        info1[p] = h.info;
+        hit = (h.info & CQX_HIT_FLAG) != 0;
+        if (hit) {
+            float pos[3], nrm[3], col[3];
+            hit_surface(d, rf_rays[i], h, pos, nrm, col);
+#pragma unroll
+            for (int a = 0; a < 3; a++) surf1[uint64_t(p) * 3 + a] = col[a];
+            uint32_t s = state[p];
+            shadow_rays(d, pos, nrm, s, sun, sky);
+        }
+    }
+    const bool take_sun = hit && d.include_sun, take_sky = hit && d.include_sky;
+    uint32_t k = wave_slot(take_sun, sh_count);
+    if (take_sun) {
+        sh_rays[k] = sun;
+        sh_tags[k] = p * 4u + kSun1;
+    }
+    k = wave_slot(take_sky, sh_count);
+    if (take_sky) {
+        sh_rays[k] = sky;
+        sh_tags[k] = p * 4u + kSky1;
+    }
+}
+
+// gatherLighting's sum from the two visibilities, in the reference's order
+__device__ __forceinline__ void gather(const DemoDev& d, const float (&n)[3], bool sun_vis, bool sky_vis,
+                                       float (&out)[3])
+{
+    out[0] = out[1] = out[2] = 0.0f;
+    if (d.include_sun && sun_vis) {
+        float dt = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; a++) dt = dt + d.sun[a] * n[a];
+        const float m = (dt < 0.0f) ? 0.0f : dt;
+#pragma unroll
+        for (int a = 0; a < 3; a++) out[a] = out[a] + 0.1f * m;
+    }
+    if (d.include_sky && sky_vis) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) out[a] = out[a] + 1.5f;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_demo_resolve(DemoDev d, uint64_t n, const uint32_t* __restrict__ info0,
+                                                      const uint32_t* __restrict__ info1,
+                                                      const float* __restrict__ surf0, const float* __restrict__ surf1,
+                                                      const uint8_t* __restrict__ vis, float* __restrict__ out)
+{
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float px[3] = { 0.8f, 0.8f, 1.0f };
+    const uint32_t i0 = info0[p];
+    if (i0 & CQX_HIT_FLAG) {
+        float n0[3], direct0[3], indirect0[3] = { 0.0f, 0.0f, 0.0f };
+        face_normal(i0, n0);
+        gather(d, n0, vis[p * 4 + kSun0] != 0, vis[p * 4 + kSky0] != 0, direct0);
+        const uint32_t i1 = info1[p];
+        if (i1 & CQX_HIT_FLAG) {
+            float n1[3], direct1[3];
+            face_normal(i1, n1);
+            gather(d, n1, vis[p * 4 + kSun1] != 0, vis[p * 4 + kSky1] != 0, direct1);
+#pragma unroll
+            for (int a = 0; a < 3; a++) indirect0[a] = surf1[p * 3 + a] * direct1[a];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float v = surf0[p * 3 + a] * (direct0[a] + indirect0[a]);
+            px[a] = d.gamma == 1.0f ? v : powf(v, d.gamma); // pow(v, 1) = v
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) out[p * 3 + a] = d.accumulate ? out[p * 3 + a] + px[a] : px[a];
+}
+
+} // namespace
+
+namespace cqx {
+
+// Per-context buffers of the mode, carved out of one allocation, for up to `cap` pixels.
+struct DemoPlan {
+    uint64_t cap = 0;
+    void* mem = nullptr;
+    float* colours = nullptr;
+    uint32_t* counts = nullptr;
+    cqx_hit* hits0 = nullptr;
+    uint32_t* state = nullptr;
+    uint32_t* info0 = nullptr;
+    uint32_t* info1 = nullptr;
+    float* surf0 = nullptr;
+    float* surf1 = nullptr;
+    uint32_t* vis = nullptr; // 4 bytes per pixel
+    cqx_ray* sh_rays = nullptr;
+    uint32_t* sh_tags = nullptr;
+    cqx_hit* sh_hits = nullptr;
+    cqx_ray* rf_rays = nullptr;
+    uint32_t* rf_tags = nullptr;
+    cqx_hit* rf_hits = nullptr;
+};
+
+void destroy_demo_plan(DemoPlan* p)
+{
+    if (!p) return;
+    release(p->mem);
+    delete p;
+}
+
+} // namespace cqx
+
+namespace {
+
+using cqx::DemoPlan;
+
+cqx_status demo_plan(cqx_ctx* ctx, uint64_t n, DemoPlan** out)
+{
+    DemoPlan* p = ctx->demo_plan;
+    if (p && p->cap >= n) {
+        *out = p;
+        return CQX_OK;
+    }
+    HIP_TRY(hipDeviceSynchronize()); // launches in flight may use the old buffers
+    cqx::destroy_demo_plan(p);
+    ctx->demo_plan = nullptr;
+    p = new (std::nothrow) DemoPlan;
+    if (!p) return fail(CQX_ERR_OOM, "out of host memory");
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t o_col = take(256 * 3 * sizeof(float)), o_cnt = take(4 * sizeof(uint32_t));
+    const size_t o_h0 = take(n * sizeof(cqx_hit)), o_st = take(n * 4), o_i0 = take(n * 4), o_i1 = take(n * 4);
+    const size_t o_s0 = take(n * 12), o_s1 = take(n * 12), o_vis = take(n * 4);
+    const size_t o_shr = take(2 * n * sizeof(cqx_ray)), o_sht = take(2 * n * 4), o_shh = take(2 * n * sizeof(cqx_hit));
+    const size_t o_rfr = take(n * sizeof(cqx_ray)), o_rft = take(n * 4), o_rfh = take(n * sizeof(cqx_hit));
+    if (hipMalloc(&p->mem, off) != hipSuccess) {
+        delete p;
+        return fail(CQX_ERR_OOM, "device allocation of %zu bytes for %llu pixels failed", off, (unsigned long long)n);
+    }
+    char* b = static_cast<char*>(p->mem);
+    p->cap = n;
+    p->colours = reinterpret_cast<float*>(b + o_col);
+    p->counts = reinterpret_cast<uint32_t*>(b + o_cnt);
+    p->hits0 = reinterpret_cast<cqx_hit*>(b + o_h0);
+    p->state = reinterpret_cast<uint32_t*>(b + o_st);
+    p->info0 = reinterpret_cast<uint32_t*>(b + o_i0);
+    p->info1 = reinterpret_cast<uint32_t*>(b + o_i1);
+    p->surf0 = reinterpret_cast<float*>(b + o_s0);
+    p->surf1 = reinterpret_cast<float*>(b + o_s1);
+    p->vis = reinterpret_cast<uint32_t*>(b + o_vis);
+    p->sh_rays = reinterpret_cast<cqx_ray*>(b + o_shr);
+    p->sh_tags = reinterpret_cast<uint32_t*>(b + o_sht);
+    p->sh_hits = reinterpret_cast<cqx_hit*>(b + o_shh);
+    p->rf_rays = reinterpret_cast<cqx_ray*>(b + o_rfr);
+    p->rf_tags = reinterpret_cast<uint32_t*>(b + o_rft);
+    p->rf_hits = reinterpret_cast<cqx_hit*>(b + o_rfh);
+    ctx->demo_plan = p;
+    *out = p;
+    return CQX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                          const cqx_demo_params* params, const cqx_launch_cfg* cfg, float* out_rgb, void* stream)
+{
+    if (!ctx || !vol || !params) return fail(CQX_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n == 0) return CQX_OK;
+    if (!rays || !out_rgb || !params->colours) return fail(CQX_ERR_INVALID_ARG, "null ray, colour or output buffer");
+    if (n > (uint64_t(1) << 30)) return fail(CQX_ERR_LIMIT, "%llu pixels exceed the 2^30 limit", (unsigned long long)n);
+    hipStream_t s = hipStream_t(stream);
+    DemoPlan* p = nullptr;
+    cqx_status st = demo_plan(ctx, n, &p);
+    if (st != CQX_OK) return st;
+    HIP_TRY(hipMemcpyAsync(p->colours, params->colours, 256 * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+
+    DemoDev d;
+    d.colours = p->colours;
+    {
+        // normalize(vec3(1, -2, 10)) as the reference computes it in FP32
+        const float v[3] = { 1.0f, -2.0f, 10.0f };
+        float sq = 0.0f;
+        for (int a = 0; a < 3; a++) sq = sq + v[a] * v[a];
+        const float len = std::sqrt(sq);
+        for (int a = 0; a < 3; a++) d.sun[a] = v[a] / len;
+    }
+    d.gamma = params->gamma != 0.0f ? params->gamma : float(1.0 / 2.2);
+    d.add_noise = params->add_noise ? 1u : 0u;
+    d.include_sun = params->include_sun ? 1u : 0u;
+    d.include_sky = params->include_sky ? 1u : 0u;
+    d.accumulate = params->accumulate ? 1u : 0u;
+    d.seed = params->seed;
+    const float fp = params->max_footprint;
+
+    // secondary launches read their ray count on the device: no binning prepass there
+    cqx_launch_cfg c2{};
+    if (cfg) c2 = *cfg;
+    c2.bin_bits = 0;
+    const uint32_t g1 = uint32_t((n + 255) / 256), g2 = uint32_t((2 * n + 255) / 256);
+    uint32_t* cnt = p->counts; // [0] first shadow rays, [1] reflected rays, [2] second shadow rays
+    hipLaunchKernelGGL(k_demo_zero, dim3(1), dim3(64), 0, s, cnt);
+    st = cqx::launch_intersect(ctx, vol, rays, n, nullptr, CQX_SURFACE_PROPERTIES, fp, p->hits0, cfg, s);
+    if (st != CQX_OK) return st;
+    hipLaunchKernelGGL(k_demo_primary, dim3(g1), dim3(256), 0, s, d, n, rays, p->hits0, p->state, p->info0, p->surf0,
+                       p->vis, p->sh_rays, p->sh_tags, cnt + 0, p->rf_rays, p->rf_tags, cnt + 1);
+    st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 0, 0u, fp, p->sh_hits, &c2, s);
+    if (st != CQX_OK) return st;
+    hipLaunchKernelGGL(k_demo_visibility, dim3(g2), dim3(256), 0, s, cnt + 0, 2 * n, p->sh_tags, p->sh_hits,
+                       reinterpret_cast<uint8_t*>(p->vis));
+    st = cqx::launch_intersect(ctx, vol, p->rf_rays, n, cnt + 1, CQX_SURFACE_PROPERTIES, fp, p->rf_hits, &c2, s);
+    if (st != CQX_OK) return st;
+    hipLaunchKernelGGL(k_demo_bounce, dim3(g1), dim3(256), 0, s, d, n, cnt + 1, p->rf_rays, p->rf_tags, p->rf_hits,
+                       p->state, p->info1, p->surf1, p->sh_rays, p->sh_tags, cnt + 2);
+    st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 2, 0u, fp, p->sh_hits, &c2, s);
+    if (st != CQX_OK) return st;
+    hipLaunchKernelGGL(k_demo_visibility, dim3(g2), dim3(256), 0, s, cnt + 2, 2 * n, p->sh_tags, p->sh_hits,
+                       reinterpret_cast<uint8_t*>(p->vis));
+    hipLaunchKernelGGL(k_demo_resolve, dim3(g1), dim3(256), 0, s, d, n, p->info0, p->info1, p->surf0, p->surf1,
+                       reinterpret_cast<const uint8_t*>(p->vis), out_rgb);
+    HIP_TRY(hipGetLastError());
+    return CQX_OK;
+}
+
+} // extern "C"

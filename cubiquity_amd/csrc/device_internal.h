@@ -19,6 +19,8 @@ namespace cqx {
 
 struct PtPlan; // path-tracer buffers and captured graph (pathtrace.hip)
 void destroy_pt_plan(PtPlan* plan);
+struct DemoPlan; // buffers of the reference-shading mode (demo.hip)
+void destroy_demo_plan(DemoPlan* plan);
 
 // Best-effort release paths: an error here has nowhere useful to go.
 inline void release(void* p) { (void)hipFree(p); }
@@ -38,6 +40,7 @@ struct cqx_ctx {
     size_t bins_bytes = 0;
     hipStream_t capture_stream = nullptr; // private stream for graph capture
     cqx::PtPlan* pt_plan = nullptr;
+    cqx::DemoPlan* demo_plan = nullptr;
     void* batch_desc = nullptr; // descriptor table of the batched launch
     size_t batch_bytes = 0;
     struct Tuned {

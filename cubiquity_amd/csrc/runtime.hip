@@ -652,6 +652,7 @@ cqx_status cqx_destroy(cqx_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     cqx::destroy_pt_plan(ctx->pt_plan);
+    cqx::destroy_demo_plan(ctx->demo_plan);
     release(ctx->d_counter);
     release(ctx->scratch);
     release(ctx->bins);

@@ -378,6 +378,38 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
                                  const cqx_launch_cfg* cfg, const float* d_image, float* d_albedo,
                                  float* d_emission, void* stream);
 
+/* ---- reference CPU-tracer shading (compatibility mode) ---------------------------------------
+ * PathtracingDemo::traceSingleRay (src/application/commands/view/pathtracing_demo.cpp:150-194)
+ * for one primary ray per pixel, with surfaceColour (:44-59, positionBasedNoise :34-42),
+ * gatherLighting (:82-117: a sun shadow ray toward normalize(1, -2, 10) and a sky shadow ray
+ * toward normalize(n + randomPointInUnitSphere()), from position + normal * 0.001) and one
+ * reflected ray from position + normal * 0.01; misses give (0.8, 0.8, 1.0); hit colours are
+ * raised to `gamma`. The FP32 arithmetic is the reference's, operation for operation, except
+ * powf (the device's, within 2 ulp).
+ * Random points: the reference draws from ONE generator shared by all pixels in scan order
+ * (pathtracing_demo.cpp:32, :61-80), which makes every pixel depend on all earlier ones. Here
+ * pixel p has its own generator state, uint32(bit_mix(seed * 2^32 + p + 1)), or 17 if that is 0,
+ * and uses the reference's draw function and draw order. Images match the reference's
+ * statistically, not bit for bit. A draw that finds no point inside the sphere in 1024 generator
+ * steps gives (0, 0, 0); the reference would loop forever. Rays the traversal flags as
+ * anomalies count as misses.
+ * rays: device, n x 6 floats (the camera rays of PathtracingDemo::raytrace, :214-225); out_rgb:
+ * device, n x 3 floats, overwritten or, with accumulate, added to (the demo's mImage +=).
+ * Asynchronous on `stream`; one call in flight per context. n <= 2^30. */
+typedef struct {
+    const float* colours; /* host, 256 x 3: the material colours (cqx_materials_colour_table) */
+    float max_footprint;  /* maxFootprint of every raycast (the demo's default: 0.0035) */
+    float gamma;          /* exponent of hit colours; 0 = the reference's 1/2.2 */
+    uint32_t add_noise;   /* the demo's addNoise (default on) */
+    uint32_t include_sun; /* includeSun (default on) */
+    uint32_t include_sky; /* includeSky (default on) */
+    uint32_t accumulate;  /* 1: out_rgb += colour, 0: out_rgb = colour */
+    uint64_t seed;        /* per-frame seed of the per-pixel generators */
+} cqx_demo_params;
+
+cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                          const cqx_demo_params* params, const cqx_launch_cfg* cfg, float* out_rgb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@
 //   fillBrush(volume, SphereBrush(c, r), material)     HostVolume::fillSphere / fillSpheres
 //                                   voxelization.h:91-126, voxelization.cpp:825-915
 //   Metadata::load (materials)      metadata.cpp:34-65 cqx_hip::loadMaterials
+//   PathtracingDemo::traceSingleRay pathtracing_demo.cpp:150-194  cqx_hip::demoTrace
 //
 // Errors become cqx_hip::Error exceptions on this side of the ABI (the C ABI itself never throws).
 #pragma once
@@ -162,6 +163,14 @@ inline std::vector<cqx_intersection> traceRayRef(const GpuVolume& vol, const std
     std::vector<cqx_intersection> out(rays.size());
     check(cqx_trace_ref_host(vol.context().get(), vol.get(), rays.data(), rays.size(), out.data()));
     return out;
+}
+
+// PathtracingDemo::traceSingleRay (pathtracing_demo.cpp:150-194) for one primary ray per pixel,
+// on device buffers: the reference CPU tracer's shading with per-pixel generator states.
+inline void demoTrace(const GpuVolume& vol, const cqx_ray* d_rays, uint64_t n, const cqx_demo_params& params,
+                      float* d_rgb, void* stream = nullptr)
+{
+    check(cqx_demo_trace(vol.context().get(), vol.get(), d_rays, n, &params, nullptr, d_rgb, stream));
 }
 
 // Materials of a volume's .toml metadata (Metadata::load, metadata.cpp:34-65), indexed by id.

@@ -39,6 +39,12 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+class _DemoParams(ctypes.Structure):
+    _fields_ = [("colours", ctypes.c_void_p), ("max_footprint", ctypes.c_float), ("gamma", ctypes.c_float),
+                ("add_noise", ctypes.c_uint32), ("include_sun", ctypes.c_uint32), ("include_sky", ctypes.c_uint32),
+                ("accumulate", ctypes.c_uint32), ("seed", ctypes.c_uint64)]
+
+
 class Oracle:
     """The C restatement (cpu_ref.c / cpu_gen.c)."""
 
@@ -65,6 +71,12 @@ class Oracle:
             "orc_compare_dense": (_U64, [_P, ctypes.c_int, _P, _U32, ctypes.c_int]),
             "orc_pathtrace": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int]),
             "orc_pathtrace_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]),
+            "orc_bit_mix": (_U64, [_U64]),
+            "orc_fnv1a": (_U64, [_P, _I64, _U64]),
+            "orc_position_noise": (ctypes.c_float, [_P]),
+            "orc_random_point": (ctypes.c_int, [_P, _P]),
+            "orc_demo_pixel_state": (_U32, [_U64, _U64]),
+            "orc_demo_trace": (None, [_P, _P, _P, _U64, _P, ctypes.c_int, _P, _P, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -113,6 +125,45 @@ class Oracle:
     def voxel(self, nodes, root, x, y, z):
         nodes = self._nodes(nodes)
         return int(self.L.orc_voxel(_ptr(nodes), root, int(x), int(y), int(z)))
+
+    # ---- the reference CPU tracer's shading (cpu_demo.c) ----
+    def bit_mix(self, x):
+        return int(self.L.orc_bit_mix(int(x) & (2**64 - 1)))
+
+    def fnv1a(self, data, seed=0xCBF29CE484222325):
+        b = np.frombuffer(bytes(data), np.uint8)
+        return int(self.L.orc_fnv1a(_ptr(b), b.size, seed))
+
+    def position_noise(self, pos):
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        return float(self.L.orc_position_noise(_ptr(p)))
+
+    def random_point(self, state):
+        """One randomPointInUnitSphere draw: (point, new state, generator steps)."""
+        st = np.array([state], np.uint32)
+        out = np.zeros(3, np.float32)
+        k = self.L.orc_random_point(_ptr(st), _ptr(out))
+        return out, int(st[0]), int(k)
+
+    def demo_pixel_state(self, seed, pixel):
+        return int(self.L.orc_demo_pixel_state(seed, pixel))
+
+    def demo_trace(self, nodes, subdags, rays, colours, max_footprint=0.0035, gamma=0.0, add_noise=True,
+                   include_sun=True, include_sky=True, seed=0, serial_state=None, threads=8, out=None):
+        """traceSingleRay per primary ray -> (n, 3) float32. serial_state: None for per-pixel
+        generators (the GPU scheme), else the reference's single generator from this state
+        (returns (image, final state))."""
+        nodes = self._nodes(nodes)
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        subdags = np.ascontiguousarray(subdags, dtype=SUBDAG_DTYPE)
+        col = np.ascontiguousarray(colours, dtype=np.float32).reshape(256, 3)
+        prm = _DemoParams(col.ctypes.data, max_footprint, gamma, int(add_noise), int(include_sun),
+                          int(include_sky), 1 if out is not None else 0, seed)
+        img = out if out is not None else np.zeros((rays.shape[0], 3), np.float32)
+        st = np.array([serial_state if serial_state is not None else 0], np.uint32)
+        self.L.orc_demo_trace(_ptr(nodes), _ptr(subdags), _ptr(rays), rays.shape[0], ctypes.byref(prm),
+                              1 if serial_state is not None else 0, _ptr(st), _ptr(img), int(threads))
+        return img if serial_state is None else (img, int(st[0]))
 
     def dense(self, nodes, root, lower, size):
         """Voxels of the box [lower, lower + size) as a [z][y][x] uint8 array."""
@@ -220,6 +271,8 @@ class Reference:
             "ref_trace_ref": (None, [_P, _P, _U64, _P]),
             "ref_time_intersect": (ctypes.c_double, [_P, _P, _U64, ctypes.c_int, ctypes.c_float, ctypes.c_int, _U64,
                                                      ctypes.c_int, _P]),
+            "ref_bit_mix": (_U64, [_U64]),
+            "ref_fnv1a": (_U64, [_P, _I64, _U64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -235,6 +288,13 @@ class Reference:
 
     def new(self):
         return RefVolume(self, self.L.ref_volume_new())
+
+    def bit_mix(self, x):
+        return int(self.L.ref_bit_mix(int(x) & (2**64 - 1)))
+
+    def fnv1a(self, data, seed=0xCBF29CE484222325):
+        b = np.frombuffer(bytes(data), np.uint8)
+        return int(self.L.ref_fnv1a(_ptr(b), b.size, seed))
 
 
 class RefVolume:

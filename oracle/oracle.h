@@ -119,6 +119,27 @@ int orc_pathtrace(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt
 int orc_pathtrace_f64(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt_params* p,
                       const double* d_image, double* image, double* d_albedo, double* d_emission, int threads);
 
+/* ---- the reference CPU tracer's shading (cpu_demo.c) ---- */
+/* same field order as the product's cqx_demo_params */
+typedef struct {
+    const float* colours; /* 256 x 3 material colours */
+    float max_footprint;
+    float gamma; /* 0 = the reference's 1/2.2 */
+    uint32_t add_noise, include_sun, include_sky, accumulate;
+    uint64_t seed;
+} orc_demo_params;
+
+uint64_t orc_bit_mix(uint64_t bits);
+uint64_t orc_fnv1a(const void* data, int64_t len, uint64_t seed);
+float orc_position_noise(const float pos[3]);
+/* one randomPointInUnitSphere draw from *state; returns the number of generator steps taken */
+int orc_random_point(uint32_t* state, float out[3]);
+uint32_t orc_demo_pixel_state(uint64_t seed, uint64_t pixel);
+/* traceSingleRay for n primary rays; out_rgb: n x 3. rng_mode 0: per-pixel states (threads may
+ * be > 1); rng_mode 1: the reference's single generator in pixel order from *serial_state. */
+void orc_demo_trace(const uint32_t* nodes, const orc_subdag* sub, const float* rays, uint64_t n,
+                    const orc_demo_params* p, int rng_mode, uint32_t* serial_state, float* out_rgb, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 //   Volume::voxel                src/library/storage.cpp:481-499
 //   Volume::setVoxel / bake      src/library/storage.cpp:396-438, 387-394
 //   NodeStore::countNodes        src/library/storage.cpp:169-190
+//   Internals::fnv1a / bit_mix   src/library/base.cpp:62-78
 // and records each call's result as plain numbers.
 #include "raytracing.h"
 #include "storage.h"
@@ -165,5 +166,9 @@ double ref_time_intersect(void* h, const float* rays, uint64_t n, int surf, floa
     if (hit_count) *hit_count = total;
     return best;
 }
+
+uint64_t ref_bit_mix(uint64_t bits) { return Internals::bit_mix(bits); }
+
+uint64_t ref_fnv1a(const void* data, int64_t length, uint64_t seed) { return Internals::fnv1a(data, length, seed); }
 
 } // extern "C"

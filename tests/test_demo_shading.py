@@ -1,0 +1,226 @@
+"""The reference CPU tracer's shading mode (PathtracingDemo::traceSingleRay,
+src/application/commands/view/pathtracing_demo.cpp:34-194): the oracle's restatement pinned
+against the compiled reference library and against properties the maths fixes (CPU), then the
+GPU entry point cqx_demo_trace against the oracle (gpu).
+
+Pinning of the oracle (cpu_demo.c):
+  * bit_mix and fnv1a: equal to the reference library's own functions (base.cpp:62-78) on random
+    inputs;
+  * randomPointInUnitSphere: each draw's generator states are the reference bit_mix chain; the
+    accepted points lie in the unit ball, the acceptance rate is pi/6, the mean is 0 and the mean
+    squared radius 3/5 (uniform in the ball);
+  * positionBasedNoise: the hash is the reference fnv1a of the rounded position; values lie on
+    the 256-level grid 0.9 + 0.1 k / 255;
+  * traceSingleRay: images with per-pixel generators agree with the reference's single shared
+    generator statistically (the per-pixel scheme changes which random numbers a pixel gets, not
+    their distribution); gamma 1 and lights off give closed-form pixels.
+The GPU bar: bit-identical to the oracle with gamma 1 (every operation is the same FP32
+operation), within 2 ulp with the reference's gamma 1/2.2 (the device powf)."""
+import numpy as np
+import pytest
+
+
+def _colours(seed=3):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.uniform(0.05, 1.0, (256, 3)).astype(np.float32)
+
+
+def test_hash_functions_match_reference(oracle, reference):
+    rng = np.random.Generator(np.random.PCG64(1))
+    xs = [int(x) for x in rng.integers(0, 2**63, 2000, dtype=np.uint64)]
+    xs += [0, 1, 17, 2**32 - 1, 2**32, 2**64 - 1]
+    assert all(oracle.bit_mix(x) == reference.bit_mix(x) for x in xs)
+    for k in range(300):
+        b = rng.integers(0, 256, int(rng.integers(0, 40)), dtype=np.uint8).tobytes()
+        seed = int(rng.integers(0, 2**63))
+        assert oracle.fnv1a(b) == reference.fnv1a(b)
+        assert oracle.fnv1a(b, seed) == reference.fnv1a(b, seed)
+
+
+def test_random_point_draws(oracle, reference):
+    state = 17  # the reference's initial generator state
+    pts, steps = [], 0
+    for _ in range(20000):
+        before = state
+        p, state, k = oracle.random_point(state)
+        # the draw walked k steps of the reference's generator
+        s = before
+        for _ in range(k):
+            s = reference.bit_mix(s) & 0xFFFFFFFF
+        assert s == state
+        pts.append(p)
+        steps += k
+    pts = np.array(pts, np.float64)
+    r2 = (pts ** 2).sum(1)
+    assert (r2 < 1.0).all()
+    assert abs(len(pts) / steps - np.pi / 6) < 0.01
+    assert np.abs(pts.mean(0)).max() < 0.02
+    assert abs(r2.mean() - 0.6) < 0.01
+    # the first accepted point from state 17, from the reference's bit_mix and the 10-bit fields
+    p0, _, k0 = oracle.random_point(17)
+    s = 17
+    for _ in range(k0):
+        s = reference.bit_mix(s) & 0xFFFFFFFF
+    f = np.array([s & 0x3FF, (s >> 10) & 0x3FF, (s >> 20) & 0x3FF], np.float32)
+    assert np.array_equal(p0, (f - np.float32(511.5)) / np.float32(511.5))
+
+
+def test_position_noise(oracle, reference):
+    rng = np.random.Generator(np.random.PCG64(2))
+    pos = rng.uniform(-3000, 3000, (3000, 3)).astype(np.float32)
+    levels = np.float32(0.9 + 0.1 * np.arange(256) / 255.0)
+    for p in pos:
+        v = np.float32(oracle.position_noise(p))
+        q = (p + np.float32(0.499)).astype(np.int32)  # truncation toward zero
+        h = reference.fnv1a(q.astype("<i4").tobytes()) & 0xFFFFFFFF
+        expect = np.float32(float(np.float32(h & 0xFF) / np.float32(255.0)) * 0.1 + 0.9)
+        assert v == expect
+        assert np.min(np.abs(levels - v)) <= 1e-7
+
+
+def _scene(cq, oracle, vol128, w=64, h=64):
+    from cubiquity_amd import workloads as W
+
+    c = W.RAYCAST_128.camera
+    rays = cq.camera_rays(c.eye, c.target, c.fov_y, w, h)
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    return rays, sub
+
+
+def test_demo_lights_off_and_misses(cq, oracle, vol128):
+    rays, sub = _scene(cq, oracle, vol128)
+    col = _colours()
+    img = oracle.demo_trace(vol128.nodes, sub, rays, col, include_sun=False, include_sky=False, gamma=1.0)
+    hits = oracle.intersect(vol128.nodes, sub, rays, True, 0.0035)
+    hit = (hits["info"] & 1).astype(bool)
+    assert 0.2 < hit.mean() < 0.95
+    assert np.array_equal(img[~hit], np.tile(np.float32([0.8, 0.8, 1.0]), ((~hit).sum(), 1)))
+    assert (img[hit] == 0).all()  # no light reaches a hit point
+
+
+def test_demo_sun_only_closed_form(cq, oracle, vol128):
+    """Sun only, no noise, gamma 1: a hit pixel is colour[m0] x (sun0 + colour[m1] x sun1), each sun
+    term 0.1 max(dot(sunDir, n), 0) when its shadow ray escapes; checked from the oracle's own
+    raycasts of the same rays."""
+    rays, sub = _scene(cq, oracle, vol128, 32, 32)
+    col = _colours()
+    img = oracle.demo_trace(vol128.nodes, sub, rays, col, include_sky=False, add_noise=False, gamma=1.0, threads=1)
+    sun = np.float32([1, -2, 10]) / np.sqrt(np.float32(105))
+    fp = np.float32(0.0035)
+    hits = oracle.intersect(vol128.nodes, sub, rays, True, fp)
+    res = oracle.to_results(rays, hits)
+    for i in np.nonzero(res["hit"])[0][:200]:
+        r0 = res[i]
+        n0 = r0["normal"].astype(np.float32)
+        so = (r0["position"] + n0 * np.float32(0.001)).astype(np.float32)
+        sh = oracle.intersect(vol128.nodes, sub, np.concatenate([so, sun])[None], False, fp)
+        dot = np.float32(np.float32(sun[0] * n0[0] + sun[1] * n0[1]) + sun[2] * n0[2])
+        d0 = np.float32(0.1) * max(dot, np.float32(0)) if not (sh["info"][0] & 1) else np.float32(0)
+        # the reflected ray draws one point from this pixel's generator
+        p, _, _ = oracle.random_point(oracle.demo_pixel_state(0, int(i)))
+        v = n0 + p
+        dirv = v / np.sqrt(np.float32(np.float32(v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]))
+        ro = r0["position"] + n0 * np.float32(0.01)
+        h1 = oracle.intersect(vol128.nodes, sub, np.concatenate([ro, dirv]).astype(np.float32)[None], True, fp)
+        r1 = oracle.to_results(np.concatenate([ro, dirv]).astype(np.float32)[None], h1)[0]
+        ind = np.zeros(3, np.float32)
+        if r1["hit"]:
+            n1 = r1["normal"].astype(np.float32)
+            so1 = (r1["position"] + n1 * np.float32(0.001)).astype(np.float32)
+            s1 = oracle.intersect(vol128.nodes, sub, np.concatenate([so1, sun])[None], False, fp)
+            dt1 = np.float32(np.float32(sun[0] * n1[0] + sun[1] * n1[1]) + sun[2] * n1[2])
+            d1 = np.float32(0.1) * max(dt1, np.float32(0)) if not (s1["info"][0] & 1) else np.float32(0)
+            ind = col[r1["material"]] * d1
+        expect = col[r0["material"]] * (d0 + ind)
+        assert np.allclose(img[i], expect, rtol=1e-6, atol=0), i
+
+
+def test_demo_per_pixel_generators_match_reference_statistically(cq, oracle, vol128):
+    """Mean over frames of the image with the reference's single shared generator against the
+    per-pixel scheme: the same picture up to Monte Carlo noise."""
+    rays, sub = _scene(cq, oracle, vol128, 48, 48)
+    col = _colours()
+    frames = 12
+    serial = np.zeros((rays.shape[0], 3), np.float32)
+    state = 17
+    for _ in range(frames):
+        img, state = oracle.demo_trace(vol128.nodes, sub, rays, col, serial_state=state)
+        serial += img
+    per_pixel = np.zeros_like(serial)
+    for f in range(frames):
+        per_pixel += oracle.demo_trace(vol128.nodes, sub, rays, col, seed=f)
+    serial /= frames
+    per_pixel /= frames
+    assert abs(float(serial.mean()) - float(per_pixel.mean())) < 0.01 * float(serial.mean())
+    assert float(np.abs(serial - per_pixel).mean()) < 0.05
+    # pixels whose primary ray misses are background in both
+    bg = (oracle.intersect(vol128.nodes, sub, rays, True, 0.0035)["info"] & 1) == 0
+    assert bg.any()
+    assert np.allclose(serial[bg], [0.8, 0.8, 1.0], rtol=1e-6) and np.array_equal(serial[bg], per_pixel[bg])
+
+
+# ---- GPU ---------------------------------------------------------------------------------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [
+    dict(),
+    dict(add_noise=False),
+    dict(include_sky=False),
+    dict(include_sun=False),
+    dict(max_footprint=-1.0),
+    dict(seed=12345),
+])
+def test_gpu_demo_matches_oracle_128(cq, oracle, gpu_ctx, vol128, opts):
+    import torch
+
+    rays, sub = _scene(cq, oracle, vol128, 128, 96)
+    col = _colours()
+    dv = gpu_ctx.upload(vol128)
+    r = torch.from_numpy(rays).cuda()
+    lin = cq.demo_trace(gpu_ctx, dv, r, col, gamma=1.0, **opts).cpu().numpy()
+    o_lin = oracle.demo_trace(vol128.nodes, sub, rays, col, gamma=1.0, **opts)
+    eq = (lin == o_lin).all(1)
+    assert eq.all(), f"{(~eq).sum()} pixels differ, first {np.nonzero(~eq)[0][:5]}"
+    img = cq.demo_trace(gpu_ctx, dv, r, col, **opts).cpu().numpy()
+    o_img = oracle.demo_trace(vol128.nodes, sub, rays, col, **opts)
+    np.testing.assert_array_max_ulp(img, o_img, maxulp=2)
+    dv.free()
+
+
+@pytest.mark.gpu
+def test_gpu_demo_accumulates_frames(cq, oracle, gpu_ctx, vol128):
+    import torch
+
+    rays, sub = _scene(cq, oracle, vol128, 64, 64)
+    col = _colours()
+    dv = gpu_ctx.upload(vol128)
+    r = torch.from_numpy(rays).cuda()
+    out = torch.zeros((rays.shape[0], 3), dtype=torch.float32, device="cuda")
+    o = np.zeros((rays.shape[0], 3), np.float32)
+    for f in range(3):
+        cq.demo_trace(gpu_ctx, dv, r, col, gamma=1.0, seed=f, out=out, accumulate=True)
+        oracle.demo_trace(vol128.nodes, sub, rays, col, gamma=1.0, seed=f, out=o)
+    assert np.array_equal(out.cpu().numpy(), o)
+    # a smaller batch after a larger one reuses the buffers
+    small = cq.demo_trace(gpu_ctx, dv, r[:1000], col, gamma=1.0, seed=9).cpu().numpy()
+    assert np.array_equal(small, oracle.demo_trace(vol128.nodes, sub, rays[:1000], col, gamma=1.0, seed=9))
+    dv.free()
+
+
+@pytest.mark.gpu
+def test_gpu_demo_1024(cq, oracle, gpu_ctx):
+    """The headline volume with its camera at 320 x 180: bit-exact at gamma 1 against the oracle."""
+    import torch
+
+    from cubiquity_amd import workloads as W
+
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    c = w.camera
+    rays = cq.camera_rays(c.eye, c.target, c.fov_y, 320, 180)
+    col = _colours(5)
+    dv = gpu_ctx.upload(hv)
+    g = cq.demo_trace(gpu_ctx, dv, torch.from_numpy(rays).cuda(), col, gamma=1.0, seed=4).cpu().numpy()
+    o = oracle.demo_trace(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, col, gamma=1.0, seed=4, threads=16)
+    assert np.array_equal(g, o)
+    dv.free()
