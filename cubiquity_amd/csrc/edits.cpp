@@ -174,6 +174,25 @@ struct SphereBrushF {
         else dist = int64_t(sum);
         return float(dist) < r2;
     }
+
+    // True when contains() is false at every point of the box [x0, x1] x [y0, y1] x [z0, z1]: the
+    // exact squared distance from the centre to the box exceeds r2 by far more than contains()'s
+    // FP32 rounding (< 1e-6 relative) and int64 truncation (1) can make up, and no point of the box
+    // is far enough away for its squared distance to leave the int64 range (contains() counts
+    // that as inside). NaN centres or radii never qualify.
+    bool box_outside(float x0, float y0, float z0, float x1, float y1, float z1) const
+    {
+        const double c[3] = { cx, cy, cz }, lo3[3] = { x0, y0, z0 }, hi3[3] = { x1, y1, z1 };
+        double dmin = 0.0, dmax = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double below = lo3[a] - c[a], above = c[a] - hi3[a];
+            const double nearest = below > 0.0 ? below : (above > 0.0 ? above : 0.0);
+            const double farthest = std::max(std::fabs(lo3[a] - c[a]), std::fabs(hi3[a] - c[a]));
+            dmin += nearest * nearest;
+            dmax += farthest * farthest;
+        }
+        return dmax < 1e18 && dmin > double(r2) * (1.0 + 1e-5) + 16.0;
+    }
 };
 
 // Node access and interning for an edit walk over the whole store (sequential edits).
@@ -287,6 +306,8 @@ struct EditWalk {
                         continue;
                     const uint32_t child = out.c[id];
                     if (child == mat) continue;
+                    // a child box wholly outside the sphere would come back unchanged from fill()
+                    if (height >= 2 && b.box_outside(fx0, fy0, fz0, fx1, fy1, fz1)) continue;
                     // all 8 corners inside (contains is a pure function: evaluation stops at the first outside)
                     const bool all = b.contains(fx0, fy0, fz0) && b.contains(fx1, fy1, fz1) && b.contains(fx0, fy0, fz1) &&
                                      b.contains(fx0, fy1, fz0) && b.contains(fx0, fy1, fz1) && b.contains(fx1, fy0, fz0) &&

@@ -49,6 +49,40 @@ def test_edits_match_reference_fillbrush(cq, oracle, reference, tmp_path, seed):
     assert np.array_equal(oracle.dense(hv.nodes, hv.root, lo, size), theirs)
 
 
+def test_edits_at_rounding_boundaries_match_reference(cq, oracle, reference, tmp_path):
+    """Spheres whose radius is within an ulp of sqrt(k) (integer squared distances sit exactly on
+    the contains() boundary), centred on integers and half-integers, and spheres far from the
+    origin where the coordinates' float spacing is coarse: the same voxels as the reference's
+    fillBrush, and the same node count after bake. These are the cases where skipping the walk
+    below a box that lies wholly outside the sphere must not change a voxel."""
+    hv = cq.HostVolume.generate(cq.gen_default_params(6, 8, n_materials=9))
+    path = str(tmp_path / "v.dag")
+    hv.save_dag(path)
+    rv = reference.load(path)
+    rng = np.random.Generator(np.random.PCG64(21))
+    edits = []
+    for i in range(160):
+        k = int(rng.integers(1, 200))
+        r = np.float32(np.sqrt(np.float32(k)))
+        r = [np.nextafter(r, np.float32(0)), r, np.nextafter(r, np.float32(100))][i % 3]
+        c = rng.integers(-4, 68, 3).astype(np.float64) + (0.5 if i % 2 else 0.0)
+        edits.append([c[0], c[1], c[2], float(r), float(rng.integers(0, 10))])
+    far = [[1.0e6 + 0.5, -3.0e6, 2.5e6, 7.3, 4.0], [-7.9e6, 1.5, 3.0, 5.0, 5.0], [33554432.0, 17.0, 1.0, 3.0, 6.0]]
+    for e in edits + far:
+        e = np.asarray(e, np.float64)
+        hv.edit_spheres(e[None], threads=1)
+        rv.fill_sphere(e[:3], e[3], int(e[4]))
+    lo, size = (-12, -12, -12), (88, 88, 88)
+    assert np.array_equal(oracle.dense(hv.nodes, hv.root, lo, size), rv.dense(lo, size))
+    for e in far:
+        x, y, z = (int(np.floor(v)) for v in e[:3])
+        for dx in (-8, -3, 0, 3, 8):
+            assert hv.voxel(x + dx, y, z) == rv.voxel(x + dx, y, z)
+    rv.bake()
+    hv.compact()
+    assert hv.node_count - 256 == rv.count_nodes()
+
+
 def test_edit_fill_whole_octant_and_carve_everything(cq, oracle, reference, tmp_path):
     """Large spheres: whole subtrees replaced by one material, then the volume emptied."""
     hv = cq.HostVolume.generate(cq.gen_default_params(5, 9))
