@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <new>
 #include <thread>
 
@@ -109,6 +110,22 @@ struct Interner {
             }
             if (std::memcmp(&nodes[slots[i]], &nodes[j], sizeof(Node)) == 0) return; // keep the first
         }
+    }
+
+    // Indexes the nodes appended to the vector since the last call (each one distinct from every
+    // indexed node), growing the table first if it would pass half full.
+    void index_appended()
+    {
+        if (2 * nodes.size() > slots.size()) {
+            size_t cap = std::max<size_t>(1024, slots.size());
+            while (cap < 2 * nodes.size()) cap *= 2;
+            slots.assign(cap, kEmpty);
+            indexed = kMaterialCount;
+        }
+        if (nodes.size() - indexed >= (size_t(1) << 16)) insert_parallel(indexed, nodes.size());
+        else
+            for (size_t j = indexed; j < nodes.size(); j++) insert_slot(uint32_t(j));
+        indexed = nodes.size();
     }
 
     // Index of a node with this content, or kEmpty (read-only: safe while no thread interns).
@@ -205,8 +222,8 @@ struct GlobalStore {
 
 // The nodes one region creates during parallel edits: a private list whose entries have the
 // temporary indices base + i (base = store size when the edits began), over a read-only view of
-// the store. A node equal to one already in the store is not created; merge() moves the list into
-// the store afterwards.
+// the store. A node equal to one already in the store is not created; merge_regions() moves the
+// lists into the store afterwards.
 struct RegionStore {
     const std::vector<Node>& nodes;
     const Interner& in;
@@ -261,19 +278,6 @@ struct RegionStore {
             }
     }
 
-    // Interns the list into the store (children before parents, as created) and returns the
-    // store index of `node` (a store index, or a temporary one of this list).
-    uint32_t merge(Interner& global, uint32_t node)
-    {
-        std::vector<uint32_t> map(local.size());
-        for (size_t i = 0; i < local.size(); i++) {
-            Node n = local[i];
-            for (int c = 0; c < 8; c++)
-                if (n.c[c] >= base) n.c[c] = map[n.c[c] - base];
-            map[i] = global.intern(n);
-        }
-        return node >= base ? map[node - base] : node;
-    }
 };
 
 template <typename Store>
@@ -332,8 +336,8 @@ struct EditWalk {
 // and the float box overlap test is monotone in the integer corners, so applying each sphere, in
 // order, to every region its bounds touch and rebuilding the levels above gives the voxels of the
 // sequential walk. Regions are independent: each applies its spheres in order into a private node
-// list (RegionStore), in parallel; the lists are merged into the store in region order, so the
-// result does not depend on the number of threads.
+// list (RegionStore), in parallel; the lists are merged into the store as if interned in region
+// order (merge_regions), so the result does not depend on the number of threads.
 
 struct RegionKey {
     uint32_t x, y, z; // region index along each axis of the offset coordinates u = v + 2^31
@@ -371,6 +375,128 @@ uint32_t region_node(const std::vector<Node>& nodes, uint32_t root, int hs, cons
         node = nodes[node].c[c];
     }
     return node;
+}
+
+// Runs f(i) for i in [0, count) on up to `threads` threads, `chunk` indices at a time.
+template <typename F>
+void parallel_for(size_t count, int threads, size_t chunk, F&& f)
+{
+    std::atomic<size_t> next{ 0 };
+    auto work = [&]() {
+        for (size_t b; (b = next.fetch_add(chunk)) < count;)
+            for (size_t i = b; i < std::min(count, b + chunk); i++) f(i);
+    };
+    const int nt = int(std::max<size_t>(1, std::min<size_t>(size_t(threads), (count + chunk - 1) / chunk)));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+}
+
+// Moves the regions' node lists into the store and maps each region's result to a store index.
+// The store is the one that interning every listed node, region by region and in list order
+// (children before parents), would give, built on all threads:
+//   * a listed node never equals a node that was in the store before the edits (one without
+//     listed children was looked up when it was created; one with a listed child points at a new
+//     node), so interning in order keeps the first of each set of equal listed nodes, their
+//     listed children replaced by their kept nodes, and appends the kept ones in order;
+//   * equal nodes are found level by level (a node's level is one more than its deepest listed
+//     child's), in a table of list positions where the lowest position wins.
+void merge_regions(std::vector<Node>& nodes, Interner& in, uint32_t base, std::vector<RegionStore>& stores,
+                   std::vector<uint32_t>& results, int threads)
+{
+    const size_t nr = stores.size();
+    std::vector<uint64_t> off(nr + 1, 0); // list position of each region's first node
+    for (size_t r = 0; r < nr; r++) off[r + 1] = off[r] + stores[r].local.size();
+    const uint64_t total = off[nr];
+    if (uint64_t(base) + total >= 0xfffffffeull) throw std::bad_alloc();
+    std::vector<uint32_t> region(total);
+    std::vector<uint8_t> level(total);
+    parallel_for(nr, threads, 64, [&](size_t r) {
+        const std::vector<Node>& loc = stores[r].local;
+        for (size_t i = 0; i < loc.size(); i++) {
+            uint32_t l = 0;
+            for (int c = 0; c < 8; c++) {
+                const uint32_t ch = loc[i].c[c];
+                if (ch >= base) l = std::max<uint32_t>(l, uint32_t(level[off[r] + (ch - base)]) + 1u);
+            }
+            region[off[r] + i] = uint32_t(r);
+            level[off[r] + i] = uint8_t(l); // at most the region height (< 32)
+        }
+    });
+    uint32_t levels = 0;
+    for (uint8_t l : level) levels = std::max<uint32_t>(levels, uint32_t(l) + 1u);
+    std::vector<uint64_t> lstart(size_t(levels) + 1, 0);
+    for (uint8_t l : level) lstart[size_t(l) + 1]++;
+    for (uint32_t l = 0; l < levels; l++) lstart[l + 1] += lstart[l];
+    std::vector<uint32_t> by_level(total); // positions grouped by level, ascending within a level
+    {
+        std::vector<uint64_t> fill(lstart.begin(), lstart.end() - 1);
+        for (uint64_t g = 0; g < total; g++) by_level[fill[level[g]]++] = uint32_t(g);
+    }
+    // Written before they are read, so left uninitialised: the kept position of each position's
+    // node, each listed node with its listed children replaced by base + their kept positions,
+    // and the slot holding each position's key.
+    std::unique_ptr<uint32_t[]> rep(new uint32_t[total]);
+    std::unique_ptr<Node[]> key(new Node[total]);
+    std::unique_ptr<uint64_t[]> slot_of(new uint64_t[total]);
+    constexpr uint32_t kFree = 0xffffffffu;
+    size_t cap = 1024;
+    while (cap < 2 * total) cap *= 2;
+    std::unique_ptr<uint32_t[]> slots(new uint32_t[cap]);
+    uint32_t* const sl = slots.get();
+    parallel_for(cap, threads, size_t(1) << 16, [&](size_t i) { sl[i] = kFree; });
+    const size_t mask = cap - 1;
+    for (uint32_t l = 0; l < levels; l++) {
+        const uint64_t b = lstart[l], e = lstart[l + 1];
+        // the key is stored before the position is published in a slot (release), and read after
+        // the slot is read (acquire)
+        parallel_for(e - b, threads, 4096, [&](size_t j) {
+            const uint32_t g = by_level[b + j];
+            const uint32_t r = region[g];
+            Node k = stores[r].local[g - off[r]];
+            for (int c = 0; c < 8; c++)
+                if (k.c[c] >= base) k.c[c] = base + rep[off[r] + (k.c[c] - base)];
+            key[g] = k;
+            for (size_t i = hash_node(k) & mask;; i = (i + 1) & mask) {
+                uint32_t cur = __atomic_load_n(&sl[i], __ATOMIC_ACQUIRE);
+                if (cur == kFree &&
+                    __atomic_compare_exchange_n(&sl[i], &cur, g, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+                    slot_of[g] = i;
+                    return;
+                }
+                // a failed exchange left the slot's position in cur
+                if (std::memcmp(&key[cur], &k, sizeof(Node)) == 0) {
+                    while (g < cur &&
+                           !__atomic_compare_exchange_n(&sl[i], &cur, g, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+                    }
+                    slot_of[g] = i;
+                    return;
+                }
+            }
+        });
+        // the lowest position of each set of equal keys is now in their slot
+        parallel_for(e - b, threads, 4096, [&](size_t j) {
+            const uint32_t g = by_level[b + j];
+            rep[g] = sl[slot_of[g]];
+        });
+    }
+    std::vector<uint32_t> fin(total); // store index of each kept position
+    uint32_t next = base;
+    for (uint64_t g = 0; g < total; g++)
+        if (rep[g] == g) fin[g] = next++;
+    nodes.resize(next);
+    parallel_for(total, threads, 4096, [&](size_t g) {
+        if (rep[g] != g) return;
+        const uint32_t r = region[g];
+        Node k = stores[r].local[g - off[r]];
+        for (int c = 0; c < 8; c++)
+            if (k.c[c] >= base) k.c[c] = fin[rep[off[r] + (k.c[c] - base)]];
+        nodes[fin[g]] = k;
+    });
+    for (size_t r = 0; r < nr; r++)
+        if (results[r] >= base) results[r] = fin[rep[off[r] + (results[r] - base)]];
+    in.index_appended();
 }
 
 // Rebuilds the levels above the regions: `regs` indexes the edited regions inside this node.
@@ -490,7 +616,7 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
         work();
         for (std::thread& t : pool) t.join();
         if (oom) throw std::bad_alloc();
-        for (size_t r = 0; r < nr; r++) results[r] = stores[r].merge(in, results[r]);
+        merge_regions(nodes, in, base, stores, results, threads);
         stores.clear();
         std::vector<uint32_t> all(nr);
         for (size_t r = 0; r < nr; r++) all[r] = uint32_t(r);

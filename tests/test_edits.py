@@ -257,3 +257,23 @@ def test_parallel_edits_fall_back_for_far_or_large_spheres(cq, oracle):
         b.edit_spheres(edits, threads=8)
         assert _store_digest(a) == _store_digest(b)
     assert b.voxel(40, 40, 40) == 0 and b.voxel(5, 7, 9) == 6
+
+
+def test_region_merge_store_is_pinned(cq):
+    """The region-parallel edit store, byte for byte: sha256 of (nodes, root) recorded from the
+    implementation that interned the regions' node lists one node at a time in region order, which
+    the parallel merge must reproduce (near-duplicate spheres across regions make equal nodes in
+    different regions)."""
+    import hashlib
+
+    from cubiquity_amd import workloads as W
+
+    for log2, seed, n, frames, digest in ((7, 3, 2000, 3, "570d28facbd52c2a"), (9, 4, 5000, 2, "0d4315e2b38dddf3")):
+        hv = cq.HostVolume.generate(cq.gen_default_params(log2, seed, n_materials=16))
+        for f in range(frames):
+            e = W.edit_frame_spheres(f, log2_size=log2, n=n)
+            if log2 == 7:
+                e[::3, :3] = e[1::3, :3] + 0.5
+            hv.edit_spheres(e, threads=8)
+        h = hashlib.sha256(np.ascontiguousarray(hv.nodes).tobytes() + str(hv.root).encode()).hexdigest()
+        assert h[:16] == digest, log2
