@@ -312,10 +312,13 @@ struct EditWalk {
                     if (child == mat) continue;
                     // a child box wholly outside the sphere would come back unchanged from fill()
                     if (height >= 2 && b.box_outside(fx0, fy0, fz0, fx1, fy1, fz1)) continue;
-                    // all 8 corners inside (contains is a pure function: evaluation stops at the first outside)
-                    const bool all = b.contains(fx0, fy0, fz0) && b.contains(fx1, fy1, fz1) && b.contains(fx0, fy0, fz1) &&
-                                     b.contains(fx0, fy1, fz0) && b.contains(fx0, fy1, fz1) && b.contains(fx1, fy0, fz0) &&
-                                     b.contains(fx1, fy0, fz1) && b.contains(fx1, fy1, fz0);
+                    // all 8 corners inside (contains is a pure function: evaluation stops at the first
+                    // outside, and a single voxel's 8 corners are one point)
+                    const bool all = ch == 0 ? b.contains(fx0, fy0, fz0)
+                                             : b.contains(fx0, fy0, fz0) && b.contains(fx1, fy1, fz1) &&
+                                                   b.contains(fx0, fy0, fz1) && b.contains(fx0, fy1, fz0) &&
+                                                   b.contains(fx0, fy1, fz1) && b.contains(fx1, fy0, fz0) &&
+                                                   b.contains(fx1, fy0, fz1) && b.contains(fx1, fy1, fz0);
                     uint32_t nc = child;
                     if (height >= 2 && !all) nc = fill(child, height - 1, x0, y0, z0);
                     else if (all) nc = mat;
