@@ -21,8 +21,40 @@ __all__ = [
     "HostVolume", "Context", "DeviceVolume", "findSubDAGs", "intersectVolume", "camera_rays",
     "incoherent_rays", "gen_default_params", "hits_to_intersections", "CqxError", "GenParams",
     "LaunchCfg", "HIT_DTYPE", "INTERSECTION_DTYPE", "SUBDAG_DTYPE", "CQX_SURFACE_PROPERTIES",
-    "MAX_FOOTPRINT_DISABLED", "decode_info", "demo_trace",
+    "MAX_FOOTPRINT_DISABLED", "decode_info", "demo_trace", "set_debug_handler", "set_warning_handler",
+    "set_progress_handler",
 ]
+
+
+_message_hooks = {}  # ctypes callbacks must outlive their registration
+
+
+def _set_hook(kind, fn, cfunc, wrap):
+    setter = getattr(lib(), f"cqx_set_{kind}_handler")
+    if fn is None:
+        setter(None)
+        _message_hooks.pop(kind, None)
+        return
+    cb = cfunc(wrap(fn))
+    setter(ctypes.cast(cb, ctypes.c_void_p))
+    _message_hooks[kind] = cb
+
+
+def set_debug_handler(fn):
+    """Cubiquity's set_debug_handler (src/library/base.h:103): fn(message: str) or None."""
+    _set_hook("debug", fn, cabi.MESSAGE_FN, lambda f: lambda m: f(m.decode()))
+
+
+def set_warning_handler(fn):
+    """Cubiquity's set_warning_handler (src/library/base.h:104): fn(message: str) or None; every
+    error the library raises is reported here as well."""
+    _set_hook("warning", fn, cabi.MESSAGE_FN, lambda f: lambda m: f(m.decode()))
+
+
+def set_progress_handler(fn):
+    """Cubiquity's setProgressHandler (src/library/base.h:106-107): fn(task: str, first_step,
+    current_step, last_step) or None."""
+    _set_hook("progress", fn, cabi.PROGRESS_FN, lambda f: lambda t, a, b, c: f(t.decode(), a, b, c))
 
 
 def gen_default_params(log2_size, seed, **overrides):

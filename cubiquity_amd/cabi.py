@@ -93,10 +93,16 @@ _U32 = ctypes.c_uint32
 _I32 = ctypes.c_int32
 _ST = ctypes.c_int32
 
+MESSAGE_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int)
+
 # name -> (restype, argtypes); the authoritative list of exported entry points.
 SIGNATURES = {
     "cqx_last_error": (ctypes.c_char_p, []),
     "cqx_version": (ctypes.c_char_p, []),
+    "cqx_set_debug_handler": (None, [_P]),
+    "cqx_set_warning_handler": (None, [_P]),
+    "cqx_set_progress_handler": (None, [_P]),
     "cqx_hv_load_dag": (_ST, [ctypes.c_char_p, _P]),
     "cqx_hv_save_dag": (_ST, [_P, ctypes.c_char_p]),
     "cqx_hv_from_nodes": (_ST, [ctypes.POINTER(ctypes.c_uint32), _U64, _U32, _P]),

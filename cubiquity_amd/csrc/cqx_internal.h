@@ -13,6 +13,11 @@ namespace cqx {
 // Records `code` and a printf-style message as the calling thread's last error; returns code.
 cqx_status fail(cqx_status code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
+// A debug note (cqx_set_debug_handler; without one, stderr when CQX_PROGRESS=1).
+void log_debug(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+// True when a debug or progress handler is set or CQX_PROGRESS=1 (progress worth computing).
+bool progress_wanted();
+void report_progress(const char* task, int first_step, int current_step, int last_step);
 
 constexpr uint32_t kMaterialCount = CQX_MATERIAL_COUNT;
 constexpr int kRootHeight = 32;

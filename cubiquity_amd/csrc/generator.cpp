@@ -199,9 +199,10 @@ public:
         std::vector<uint32_t> task_ref(ntasks);
         std::atomic<int64_t> next{ 0 }, done{ 0 };
         std::mutex merge_lock;
-        // CQX_PROGRESS=1: a line on stderr each time another tenth of the regions is built
-        const char* env = std::getenv("CQX_PROGRESS");
-        const bool progress = env && env[0] == '1';
+        // progress (cqx_set_progress_handler / debug notes) each time another tenth of the regions
+        // is built
+        const bool progress = progress_wanted();
+        if (progress) report_progress("Building volume", 0, 0, int(ntasks));
         auto work = [&]() {
             NodeTable local;
             std::vector<uint32_t> remap;
@@ -227,8 +228,9 @@ public:
                 const int64_t k = ++done;
                 if (progress && (k * 10) / ntasks != ((k - 1) * 10) / ntasks) {
                     std::lock_guard<std::mutex> g(merge_lock);
-                    std::fprintf(stderr, "[cqx] built %lld of %lld regions, %zu nodes\n", (long long)k,
-                                 (long long)ntasks, global.nodes.size());
+                    log_debug("built %lld of %lld regions, %zu nodes", (long long)k, (long long)ntasks,
+                              global.nodes.size());
+                    report_progress("Building volume", 0, int(k), int(ntasks));
                 }
             }
         };
@@ -538,8 +540,7 @@ cqx_status cqx_hv_generate(const cqx_gen_params* p, int threads, cqx_host_volume
         uint32_t root = g.intern(r);
         if (g.nodes.size() >= 0xffffffffull) return fail(CQX_ERR_LIMIT, "volume needs more than 2^32 nodes");
         g.drop_index();
-        const char* env = std::getenv("CQX_PROGRESS");
-        if (env && env[0] == '1') std::fprintf(stderr, "[cqx] renumbering %zu nodes\n", g.nodes.size());
+        log_debug("renumbering %zu nodes", g.nodes.size());
         canonical_order(g.nodes, root);
         cqx_host_volume* hv = new cqx_host_volume;
         hv->v.nodes = std::move(g.nodes);

@@ -2,16 +2,27 @@
 // expansion of compact GPU records into the reference's RayVolumeIntersection fields.
 #include "cqx_internal.h"
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 
 namespace cqx {
 
 static thread_local std::string g_last_error;
+static std::atomic<cqx_message_fn> g_debug_handler{ nullptr }, g_warning_handler{ nullptr };
+static std::atomic<cqx_progress_fn> g_progress_handler{ nullptr };
+static std::mutex g_handler_lock; // one handler call at a time
+
+static bool progress_env()
+{
+    const char* env = std::getenv("CQX_PROGRESS");
+    return env && env[0] == '1';
+}
 
 cqx_status fail(cqx_status code, const char* fmt, ...)
 {
@@ -21,7 +32,35 @@ cqx_status fail(cqx_status code, const char* fmt, ...)
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
     g_last_error = buf;
+    if (cqx_message_fn f = g_warning_handler.load()) {
+        std::lock_guard<std::mutex> g(g_handler_lock);
+        f(buf);
+    }
     return code;
+}
+
+void log_debug(const char* fmt, ...)
+{
+    cqx_message_fn f = g_debug_handler.load();
+    if (!f && !progress_env()) return;
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    std::lock_guard<std::mutex> g(g_handler_lock);
+    if (f) f(buf);
+    else std::fprintf(stderr, "[cqx] %s\n", buf);
+}
+
+bool progress_wanted() { return g_debug_handler.load() || g_progress_handler.load() || progress_env(); }
+
+void report_progress(const char* task, int first_step, int current_step, int last_step)
+{
+    if (cqx_progress_fn f = g_progress_handler.load()) {
+        std::lock_guard<std::mutex> g(g_handler_lock);
+        f(task, first_step, current_step, last_step);
+    }
 }
 
 void clear_error() { g_last_error.clear(); }
@@ -140,6 +179,10 @@ using namespace cqx;
 extern "C" {
 
 const char* cqx_last_error(void) { return g_last_error.c_str(); }
+
+void cqx_set_debug_handler(cqx_message_fn fn) { g_debug_handler.store(fn); }
+void cqx_set_warning_handler(cqx_message_fn fn) { g_warning_handler.store(fn); }
+void cqx_set_progress_handler(cqx_progress_fn fn) { g_progress_handler.store(fn); }
 
 const char* cqx_version(void) { return "cqx 0.1 (gfx950)"; }
 

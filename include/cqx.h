@@ -99,6 +99,22 @@ typedef struct {
 const char* cqx_last_error(void);
 const char* cqx_version(void);
 
+/* ---- messages --------------------------------------------------------------------------------
+ * Cubiquity's set_debug_handler / set_warning_handler / setProgressHandler
+ * (src/library/base.h:102-107, base.cpp:86-131): process-wide hooks, NULL by default.
+ *   warning   every error as it is raised (the text cqx_last_error() then returns on that thread)
+ *   debug     notes from long operations (volume generation progress with node counts,
+ *             renumbering); without a debug handler, CQX_PROGRESS=1 in the environment prints
+ *             them on stderr
+ *   progress  (task, first_step, current_step, last_step) from long operations, the first call
+ *             at first_step and the last at last_step (volume generation: regions built)
+ * Handlers may be called from the library's worker threads; calls are serialised. */
+typedef void (*cqx_message_fn)(const char* message);
+typedef void (*cqx_progress_fn)(const char* task, int first_step, int current_step, int last_step);
+void cqx_set_debug_handler(cqx_message_fn fn);
+void cqx_set_warning_handler(cqx_message_fn fn);
+void cqx_set_progress_handler(cqx_progress_fn fn);
+
 /* ---- host node store (replaces Volume/NodeStore access on the path) --------------------------- */
 
 /* Nodes are stored as in the reference's NodeStore: node_count x uint32[8], indices 0..255 are
