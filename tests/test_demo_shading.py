@@ -224,3 +224,22 @@ def test_gpu_demo_1024(cq, oracle, gpu_ctx):
     o = oracle.demo_trace(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, col, gamma=1.0, seed=4, threads=16)
     assert np.array_equal(g, o)
     dv.free()
+
+
+@pytest.mark.gpu
+def test_gpu_demo_empty_and_argument_errors(cq, gpu_ctx, vol128):
+    import ctypes
+
+    import torch
+
+    dv = gpu_ctx.upload(vol128)
+    empty = torch.zeros((0, 6), dtype=torch.float32, device="cuda")
+    assert tuple(cq.demo_trace(gpu_ctx, dv, empty, _colours()).shape) == (0, 3)
+    r = torch.zeros((4, 6), dtype=torch.float32, device="cuda")
+    o = torch.zeros((4, 3), dtype=torch.float32, device="cuda")
+    prm = cq.cabi.DemoParams(None, 0.0035, 0.0, 1, 1, 1, 0, 0)  # no colour table
+    st = cq.cabi.lib().cqx_demo_trace(gpu_ctx._h, dv._h, ctypes.c_void_p(r.data_ptr()), 4, ctypes.byref(prm), None,
+                                      ctypes.c_void_p(o.data_ptr()), None)
+    assert st == -1  # CQX_ERR_INVALID_ARG
+    torch.cuda.synchronize()
+    dv.free()
