@@ -190,7 +190,10 @@ def bench_edit_frames(cq, W, ctx, frames):
     dv = ctx.upload(hv)
     nodes0 = hv.node_count
     trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
+    # node entries per ray of the last launch (record word 6), summed on the GPU
+    entries = lambda: int(hits.view(torch.int32)[:, 6].to(torch.int64).sum().item()) / rays.shape[0]  # noqa: E731
     before = timed(trace, 3)
+    entries_before = entries()
     edit_ms, update_ms, trace_ms, appended = [], [], [], []
     for f in range(frames):
         e = W.edit_frame_spheres(f)
@@ -202,13 +205,24 @@ def bench_edit_frames(cq, W, ctx, frames):
         update_ms.append((time.perf_counter() - t) * 1e3)
         appended.append(hv.node_count - first)
         trace_ms.append(timed(trace, 3))
+    entries_after = entries()
+    # the same edited volume compacted (bake: breadth-first order, no appended ranges), uploaded again
+    nodes_edited = hv.node_count
+    hv.compact()
+    dv.free()
+    dv = ctx.upload(hv)
+    compacted = timed(trace, 3)
     dv.free()
     mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
     return {"volume": "2048^3 seed 21, 16 materials", "edits_per_frame": W.EDIT_FRAME_EDITS, "frames": frames,
-            "build_s": round(build_s, 2), "nodes_before": nodes0, "nodes_after": hv.node_count,
+            "build_s": round(build_s, 2), "nodes_before": nodes0, "nodes_after": nodes_edited,
+            "nodes_after_compaction": hv.node_count,
             "host_edit_ms": mean(edit_ms), "edits_per_s": round(W.EDIT_FRAME_EDITS / (np.mean(edit_ms) * 1e-3)),
             "appended_nodes_per_frame": int(np.mean(appended)), "device_update_ms": mean(update_ms),
             "trace_ms_before_edits": round(before, 3), "trace_ms": mean(trace_ms),
+            "node_entries_per_ray_before_edits": round(entries_before, 3),
+            "node_entries_per_ray_after_edits": round(entries_after, 3),
+            "trace_ms_after_compaction": round(compacted, 3),
             "frame_ms": round(float(np.mean(edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 3)}
 
 
@@ -253,7 +267,11 @@ def bench_large(cq, W, ctx, frames, w=None):
     best, _ = cq.autotune(ctx, dv, rays, True)
     trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
     before = timed(trace, 2)
-    log(f"large: traversal {before:.2f} ms")
+    # node entries and hits per ray of the last launch (record words 6 and 5 bit 0), summed on the GPU
+    per_ray = lambda: (int(hits.view(torch.int32)[:, 6].to(torch.int64).sum().item()) / rays.shape[0],  # noqa: E731
+                       float((hits.view(torch.int32)[:, 5] & 1).to(torch.float64).mean().item()))
+    entries0, hitfrac0 = per_ray()
+    log(f"large: traversal {before:.2f} ms, {entries0:.2f} node entries per ray, hit fraction {hitfrac0:.4f}")
     nodes0 = hv.node_count
     edit_ms, update_ms, trace_ms, appended = [], [], [], []
     for f in range(frames):
@@ -266,7 +284,9 @@ def bench_large(cq, W, ctx, frames, w=None):
         update_ms.append((time.perf_counter() - t) * 1e3)
         appended.append(hv.node_count - first)
         trace_ms.append(timed(trace, 2))
-        log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms, update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms")
+        entries1, hitfrac1 = per_ray()
+        log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms, update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms, "
+            f"{entries1:.2f} node entries per ray, hit fraction {hitfrac1:.4f}")
     dv.free()
     mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
     n = rays.shape[0]
@@ -277,6 +297,8 @@ def bench_large(cq, W, ctx, frames, w=None):
             "host_edit_ms_steady": mean(edit_ms[1:]) if len(edit_ms) > 1 else mean(edit_ms),
             "appended_nodes_per_frame": int(np.mean(appended)), "device_update_ms": mean(update_ms),
             "trace_ms_after_edits": mean(trace_ms),
+            "node_entries_per_ray": round(entries0, 3), "node_entries_per_ray_after_edits": round(entries1, 3),
+            "hit_fraction": round(hitfrac0, 5), "hit_fraction_after_edits": round(hitfrac1, 5),
             "frame_ms_steady": round(float(np.mean(edit_ms[1:] or edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 1),
             "launch": {f2: getattr(best, f2) for f2, _ in best._fields_}}
 
