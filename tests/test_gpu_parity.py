@@ -261,6 +261,36 @@ def test_gpu_host_entry_point(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
     assert records_equal(g, o).all()
 
 
+def test_gpu_store_beyond_2_27_nodes(cq, oracle, gpu_ctx, vol128, rays128):
+    """Stores of more than 2^27 nodes (the 16384^3 workload has 910M) read children with 64-bit
+    indexing instead of 32-bit byte offsets. The 128^3 store padded with 2^27 unreachable nodes
+    (4.3 GB) takes that path; every kernel variant, binned and not, and traceRayRef must give the
+    oracle's records on the unpadded store."""
+    import torch
+
+    base = vol128.nodes
+    nodes = np.zeros((base.shape[0] + (1 << 27), 8), np.uint32)  # padding nodes hold material 0
+    nodes[: base.shape[0]] = base
+    dv = gpu_ctx.upload((nodes, vol128.root))
+    del nodes
+    assert dv.node_count > (1 << 27)
+    sub = oracle.find_subdags(base, vol128.root)
+    for set_name in ("camera", "incoherent", "special"):
+        rays = rays128[set_name]
+        o = oracle.intersect(base, sub, rays, True, -1.0, threads=8)
+        for cfg in (cq.LaunchCfg(variant=1), cq.LaunchCfg(variant=2), cq.LaunchCfg(variant=3),
+                    cq.LaunchCfg(variant=4, lds_nodes=512), cq.LaunchCfg(variant=5),
+                    cq.LaunchCfg(variant=3, bin_bits=3)):
+            g = _gpu(gpu_ctx, dv, rays, True, -1.0, cfg)
+            eq = records_equal(g, o)
+            assert eq.all(), _report(f"{set_name} {cfg.variant}/{cfg.bin_bits}", eq, rays, g, o)
+    rays = np.concatenate([rays128["camera"][::16], rays128["special"]])
+    got = cq.traceRayRef(gpu_ctx, dv, torch.from_numpy(np.ascontiguousarray(rays, dtype=np.float32)).cuda())
+    torch.cuda.synchronize()
+    assert got.cpu().numpy().tobytes() == oracle.trace_ref(base, vol128.root, rays).tobytes()
+    dv.free()
+
+
 def test_gpu_trace_ref_matches_oracle(cq, oracle, gpu_ctx, dvol128, vol128, rays128):
     """cqx_trace_ref (the reference's brute-force raycast) against the oracle's restatement, byte
     for byte, on the 128^3 ray sets and on volumes with 21-22-level SubDAGs."""
