@@ -166,7 +166,8 @@ class HostVolume:
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None
-        if h and cabi.loaded():
+        # at interpreter exit the module globals may already be gone: the process frees the memory
+        if h and cabi is not None and cabi.loaded():
             lib().cqx_hv_free(h)
 
 

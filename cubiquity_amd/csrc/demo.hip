@@ -17,7 +17,8 @@
 //   noise(pos) = float(0.9 + 0.1 x ((fnv1a(int(pos + 0.499)) & 0xff) / 255)) (the mapping in
 //         double: the reference's literals are doubles)               positionBasedNoise, :34-42
 //   randomPointInUnitSphere: s = uint32(bit_mix(s)); the three 10-bit fields of s mapped by
-//         (v - 511.5) / 511.5, repeated while the squared length is >= 1                   :61-80
+//         (v - 511.5) / 511.5 (tabulated per block), repeated while the squared length is >= 1
+//                                                                                      :61-80
 // The reference draws from ONE generator state shared by all pixels in scan order, so a pixel's
 // draws depend on how many draws every earlier pixel made (which depends on their hits). Here each
 // pixel has its own state, uint32(bit_mix(seed x 2^32 + p + 1)) (17, the reference's initial
@@ -90,19 +91,40 @@ __device__ __forceinline__ float position_noise(const float (&pos)[3])
     return float((double(noise) * 0.1) + 0.9);
 }
 
-__device__ __forceinline__ void random_point(uint32_t& s, float (&r)[3])
+// A coordinate of a random point is (f - 511.5) / 511.5 for a 10-bit field f: 1024 possible FP32
+// values. Each block tabulates them and their FP32 squares in LDS once, so a draw's rejection test
+// is three lookups and two adds, with the same values the arithmetic gives.
+struct PointTable {
+    float v[1024];
+    float sq[1024];
+};
+
+// Called by every thread of the block (it synchronises the block).
+__device__ __forceinline__ void fill_point_table(PointTable& t)
+{
+    for (uint32_t i = threadIdx.x; i < 1024; i += blockDim.x) {
+        const float v = (float(i) - 511.5f) / 511.5f;
+        t.v[i] = v;
+        t.sq[i] = v * v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void random_point(uint32_t& s, float (&r)[3], const PointTable& t)
 {
     for (int k = 0; k < kMaxDraws; k++) {
         s = uint32_t(bit_mix(s));
-        r[0] = float(s & 0x3FFu);
-        r[1] = float((s >> 10) & 0x3FFu);
-        r[2] = float((s >> 20) & 0x3FFu);
-#pragma unroll
-        for (int a = 0; a < 3; a++) r[a] = (r[a] - 511.5f) / 511.5f;
+        const uint32_t f0 = s & 0x3FFu, f1 = (s >> 10) & 0x3FFu, f2 = (s >> 20) & 0x3FFu;
         float d = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 3; a++) d = d + r[a] * r[a];
-        if (!(d >= 1.0f)) return;
+        d = d + t.sq[f0];
+        d = d + t.sq[f1];
+        d = d + t.sq[f2];
+        if (!(d >= 1.0f)) {
+            r[0] = t.v[f0];
+            r[1] = t.v[f1];
+            r[2] = t.v[f2];
+            return;
+        }
     }
     r[0] = r[1] = r[2] = 0.0f;
 }
@@ -154,7 +176,7 @@ __device__ __forceinline__ void hit_surface(const DemoDev& d, const cqx_ray& r, 
 
 // The shadow rays of gatherLighting at (pos, n); the sky direction takes one draw from s.
 __device__ __forceinline__ void shadow_rays(const DemoDev& d, const float (&pos)[3], const float (&n)[3], uint32_t& s,
-                                            cqx_ray& sun, cqx_ray& sky)
+                                            const PointTable& pts, cqx_ray& sun, cqx_ray& sky)
 {
     float so[3];
 #pragma unroll
@@ -167,7 +189,7 @@ __device__ __forceinline__ void shadow_rays(const DemoDev& d, const float (&pos)
     }
     if (d.include_sky) {
         float pt[3], v[3], dir[3];
-        random_point(s, pt);
+        random_point(s, pt, pts);
 #pragma unroll
         for (int a = 0; a < 3; a++) v[a] = n[a] + pt[a];
         normalize3(v, dir);
@@ -205,6 +227,8 @@ __global__ void __launch_bounds__(256) k_demo_primary(DemoDev d, uint64_t n, con
                                                       cqx_ray* __restrict__ rf_rays, uint32_t* __restrict__ rf_tags,
                                                       uint32_t* __restrict__ rf_count)
 {
+    __shared__ PointTable pts;
+    fill_point_table(pts);
     const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     bool hit = false;
     cqx_ray sun, sky, refl;
@@ -219,9 +243,9 @@ __global__ void __launch_bounds__(256) k_demo_primary(DemoDev d, uint64_t n, con
 #pragma unroll
             for (int a = 0; a < 3; a++) surf0[p * 3 + a] = col[a];
             uint32_t s = pixel_state(d.seed, p);
-            shadow_rays(d, pos, nrm, s, sun, sky);
+            shadow_rays(d, pos, nrm, s, pts, sun, sky);
             float pt[3], v[3], dir[3];
-            random_point(s, pt);
+            random_point(s, pt, pts);
 #pragma unroll
             for (int a = 0; a < 3; a++) v[a] = nrm[a] + pt[a];
             normalize3(v, dir);
@@ -271,6 +295,8 @@ __global__ void __launch_bounds__(256) k_demo_bounce(DemoDev d, uint64_t cap, co
                                                      cqx_ray* __restrict__ sh_rays, uint32_t* __restrict__ sh_tags,
                                                      uint32_t* __restrict__ sh_count)
 {
+    __shared__ PointTable pts;
+    fill_point_table(pts);
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint64_t c = *rf_count;
     bool hit = false;
@@ -287,7 +313,7 @@ __global__ void __launch_bounds__(256) k_demo_bounce(DemoDev d, uint64_t cap, co
 #pragma unroll
             for (int a = 0; a < 3; a++) surf1[uint64_t(p) * 3 + a] = col[a];
             uint32_t s = state[p];
-            shadow_rays(d, pos, nrm, s, sun, sky);
+            shadow_rays(d, pos, nrm, s, pts, sun, sky);
         }
     }
     const bool take_sun = hit && d.include_sun, take_sky = hit && d.include_sky;
