@@ -85,6 +85,31 @@ def cpu_baseline(hv, rays, hits_gpu, cq):
     }
 
 
+def cpu_reference_shading(cq, W):
+    """The reference CPU path tracer's frame (PathtracingDemo::raytrace / traceSingleRay: one
+    thread and one random-point generator shared by all pixels in scan order, as the demo runs it)
+    for the 1280x720 camera of the 1024^3 volume, on the oracle's restatement, whose traversal is
+    bit-identical to the reference's. Same frame as the GPU reference-shading section."""
+    from oracle.bindings import Oracle
+
+    orc = Oracle()
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    rays = cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, 1280, 720)
+    alb, _ = cq.material_table(9, 11)
+    colours = np.zeros((256, 3), np.float32)
+    colours[1:10] = alb
+    sub = orc.find_subdags(hv.nodes, hv.root)
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        orc.demo_trace(hv.nodes, sub, rays, colours, serial_state=17)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return {"frame_ms": round(best * 1e3, 1), "cores": 1, "kind": "port",
+            "sample": "the whole 1280x720 frame, LOD 0.0035, sun + sky + one reflection, best of 2"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -526,6 +551,10 @@ def main():
         log(f"edit frames: {out['edit_frames_2048']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(hv, rays, hits, cq)
+        ref_frame = cpu_reference_shading(cq, W)
+        if "pathtrace_1024" in out:
+            ref_frame["gpu_speedup"] = round(ref_frame["frame_ms"] / out["pathtrace_1024"]["reference_shading_frame_ms"], 1)
+        out["cpu_baseline"]["reference_shading_frame"] = ref_frame
         out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
