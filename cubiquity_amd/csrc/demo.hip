@@ -28,7 +28,7 @@
 // Rays the traversal flags as anomalies (the reference would not return) count as misses.
 //
 // Wavefront: trace primaries -> k_demo_primary (sun/sky shadow rays and the reflected ray, each
-// list compacted by one atomic per wave) -> trace shadow rays -> k_demo_visibility -> trace
+// list compacted by one atomic per 1024-thread block) -> trace shadow rays -> k_demo_visibility -> trace
 // reflected rays -> k_demo_bounce (second-hit shadow rays) -> trace -> k_demo_visibility ->
 // k_demo_resolve. Ray counts stay on the device; the persistent traversal kernel reads them.
 // Visibility is kept per (pixel, shadow ray) and summed in the reference's order at the end.
@@ -198,20 +198,6 @@ __device__ __forceinline__ void shadow_rays(const DemoDev& d, const float (&pos)
     }
 }
 
-// Slot of this lane's entry in a list compacted by one atomic per wave (call with the whole wave).
-__device__ __forceinline__ uint32_t wave_slot(bool take, uint32_t* count)
-{
-    const uint64_t mask = __ballot(take);
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (mask) {
-        const int leader = __ffsll((long long)mask) - 1;
-        if (int(lane) == leader) base = atomicAdd(count, uint32_t(__popcll(mask)));
-        base = __shfl(base, leader);
-    }
-    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-}
-
 __global__ void k_demo_zero(uint32_t* __restrict__ counts)
 {
     if (threadIdx.x < 4) counts[threadIdx.x] = 0u;
@@ -219,7 +205,7 @@ __global__ void k_demo_zero(uint32_t* __restrict__ counts)
 
 // First hits: surface colour, the two shadow rays and the reflected ray; clears the pixel's
 // visibility bytes.
-__global__ void __launch_bounds__(256) k_demo_primary(DemoDev d, uint64_t n, const cqx_ray* __restrict__ rays,
+__global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_primary(DemoDev d, uint64_t n, const cqx_ray* __restrict__ rays,
                                                       const cqx_hit* __restrict__ hits, uint32_t* __restrict__ state,
                                                       uint32_t* __restrict__ info0, float* __restrict__ surf0,
                                                       uint32_t* __restrict__ vis, cqx_ray* __restrict__ sh_rays,
@@ -257,21 +243,21 @@ __global__ void __launch_bounds__(256) k_demo_primary(DemoDev d, uint64_t n, con
             state[p] = s;
         }
     }
-    const bool take_sun = hit && d.include_sun, take_sky = hit && d.include_sky;
-    uint32_t k = wave_slot(take_sun, sh_count);
-    if (take_sun) {
-        sh_rays[k] = sun;
-        sh_tags[k] = uint32_t(p) * 4u + kSun0;
+    const bool take[3] = { hit && d.include_sun, hit && d.include_sky, hit };
+    uint32_t* const counts[3] = { sh_count, sh_count, rf_count };
+    uint32_t k[3];
+    cqx::block_slots<cqx::kCompactBlock>(take, counts, k);
+    if (take[0]) {
+        sh_rays[k[0]] = sun;
+        sh_tags[k[0]] = uint32_t(p) * 4u + kSun0;
     }
-    k = wave_slot(take_sky, sh_count);
-    if (take_sky) {
-        sh_rays[k] = sky;
-        sh_tags[k] = uint32_t(p) * 4u + kSky0;
+    if (take[1]) {
+        sh_rays[k[1]] = sky;
+        sh_tags[k[1]] = uint32_t(p) * 4u + kSky0;
     }
-    k = wave_slot(hit, rf_count);
-    if (hit) {
-        rf_rays[k] = refl;
-        rf_tags[k] = uint32_t(p);
+    if (take[2]) {
+        rf_rays[k[2]] = refl;
+        rf_tags[k[2]] = uint32_t(p);
     }
 }
 
@@ -287,7 +273,7 @@ __global__ void __launch_bounds__(256) k_demo_visibility(const uint32_t* __restr
 }
 
 // Second hits (the reflected rays): surface colour and the second gatherLighting's shadow rays.
-__global__ void __launch_bounds__(256) k_demo_bounce(DemoDev d, uint64_t cap, const uint32_t* __restrict__ rf_count,
+__global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_bounce(DemoDev d, uint64_t cap, const uint32_t* __restrict__ rf_count,
                                                      const cqx_ray* __restrict__ rf_rays,
                                                      const uint32_t* __restrict__ rf_tags,
                                                      const cqx_hit* __restrict__ rf_hits, const uint32_t* __restrict__ state,
@@ -316,16 +302,17 @@ __global__ void __launch_bounds__(256) k_demo_bounce(DemoDev d, uint64_t cap, co
             shadow_rays(d, pos, nrm, s, pts, sun, sky);
         }
     }
-    const bool take_sun = hit && d.include_sun, take_sky = hit && d.include_sky;
-    uint32_t k = wave_slot(take_sun, sh_count);
-    if (take_sun) {
-        sh_rays[k] = sun;
-        sh_tags[k] = p * 4u + kSun1;
+    const bool take[2] = { hit && d.include_sun, hit && d.include_sky };
+    uint32_t* const counts[2] = { sh_count, sh_count };
+    uint32_t k[2];
+    cqx::block_slots<cqx::kCompactBlock>(take, counts, k);
+    if (take[0]) {
+        sh_rays[k[0]] = sun;
+        sh_tags[k[0]] = p * 4u + kSun1;
     }
-    k = wave_slot(take_sky, sh_count);
-    if (take_sky) {
-        sh_rays[k] = sky;
-        sh_tags[k] = p * 4u + kSky1;
+    if (take[1]) {
+        sh_rays[k[1]] = sky;
+        sh_tags[k[1]] = p * 4u + kSky1;
     }
 }
 
@@ -507,11 +494,12 @@ cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* r
     if (cfg) c2 = *cfg;
     c2.bin_bits = 0;
     const uint32_t g1 = uint32_t((n + 255) / 256), g2 = uint32_t((2 * n + 255) / 256);
+    const uint32_t gc = uint32_t((n + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
     uint32_t* cnt = p->counts; // [0] first shadow rays, [1] reflected rays, [2] second shadow rays
     hipLaunchKernelGGL(k_demo_zero, dim3(1), dim3(64), 0, s, cnt);
     st = cqx::launch_intersect(ctx, vol, rays, n, nullptr, CQX_SURFACE_PROPERTIES, fp, p->hits0, cfg, s);
     if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_primary, dim3(g1), dim3(256), 0, s, d, n, rays, p->hits0, p->state, p->info0, p->surf0,
+    hipLaunchKernelGGL(k_demo_primary, dim3(gc), dim3(cqx::kCompactBlock), 0, s, d, n, rays, p->hits0, p->state, p->info0, p->surf0,
                        p->vis, p->sh_rays, p->sh_tags, cnt + 0, p->rf_rays, p->rf_tags, cnt + 1);
     st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 0, 0u, fp, p->sh_hits, &c2, s);
     if (st != CQX_OK) return st;
@@ -519,7 +507,7 @@ cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* r
                        reinterpret_cast<uint8_t*>(p->vis));
     st = cqx::launch_intersect(ctx, vol, p->rf_rays, n, cnt + 1, CQX_SURFACE_PROPERTIES, fp, p->rf_hits, &c2, s);
     if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_bounce, dim3(g1), dim3(256), 0, s, d, n, cnt + 1, p->rf_rays, p->rf_tags, p->rf_hits,
+    hipLaunchKernelGGL(k_demo_bounce, dim3(gc), dim3(cqx::kCompactBlock), 0, s, d, n, cnt + 1, p->rf_rays, p->rf_tags, p->rf_hits,
                        p->state, p->info1, p->surf1, p->sh_rays, p->sh_tags, cnt + 2);
     st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 2, 0u, fp, p->sh_hits, &c2, s);
     if (st != CQX_OK) return st;

@@ -1,4 +1,4 @@
-// Internal declarations shared by the HIP sources of libcqx.so (runtime.hip, pathtrace.hip).
+// Internal declarations shared by the HIP sources of libcqx.so.
 #pragma once
 
 #include "cqx_internal.h"
@@ -74,5 +74,47 @@ cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes);
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                             const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
                             const cqx_launch_cfg* cfg, hipStream_t stream);
+
+// Threads per block of the kernels that compact lists with block_slots.
+constexpr uint32_t kCompactBlock = 1024;
+
+// Stream compaction into N lists at once: a thread with take[l] set gets slot[l], its entry's
+// index in list l. Each list's slots for the whole block are claimed with one atomicAdd on
+// count[l] (counts may be shared: every list claims its own range). One atomic per wave on a
+// counter that every block uses serialises at the memory side (several nanoseconds each), which
+// made it the largest cost of a shading pass over millions of rays; per block it is 16 times
+// rarer. Entries keep no order across blocks. Call once per kernel, from every thread of a
+// kBlock-thread block (it synchronises the block).
+template <uint32_t kBlock, int N>
+__device__ __forceinline__ void block_slots(const bool (&take)[N], uint32_t* const (&count)[N], uint32_t (&slot)[N])
+{
+    static_assert(kBlock % 64 == 0 && kBlock <= 1024, "whole waves, at most 16");
+    constexpr uint32_t kWaves = kBlock / 64;
+    __shared__ uint32_t part[N][kWaves + 1]; // per-wave counts, then their prefix; [kWaves]: the block's base
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t mask[N];
+#pragma unroll
+    for (int l = 0; l < N; l++) {
+        mask[l] = __ballot(take[l]);
+        if (lane == 0) part[l][wave] = uint32_t(__popcll(mask[l]));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < N; l++) {
+        if (threadIdx.x == uint32_t(l)) { // list l's prefix and claim, on thread l
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < kWaves; w++) {
+                const uint32_t c = part[l][w];
+                part[l][w] = sum;
+                sum += c;
+            }
+            part[l][kWaves] = sum ? atomicAdd(count[l], sum) : 0u;
+        }
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int l = 0; l < N; l++) slot[l] = part[l][kWaves] + part[l][wave] + uint32_t(__popcll(mask[l] & below));
+}
 
 } // namespace cqx

@@ -16,7 +16,8 @@
 // Forward, two forms with identical per-path arithmetic:
 //   wavefront  generate -> (intersect, shade) x (bounces + 1), captured once in a hipGraph per
 //              plan and replayed; ray counts live on the device, the persistent traversal kernel
-//              reads them; surviving rays are compacted by one atomic per wave
+//              reads them; surviving rays are compacted by one atomic per
+//              1024-thread block
 //   megakernel one lane per path, grid-stride
 // Both write per-path radiance; k_pt_resolve sums the samples of a pixel in sample order.
 // Adjoint (path replay): regenerates each path from its RNG (nothing is stored by the forward
@@ -193,7 +194,7 @@ __global__ void k_set_u32(uint32_t* __restrict__ x, uint32_t v)
     if (threadIdx.x == 0) *x = v;
 }
 
-__global__ void __launch_bounds__(256) k_pt_shade(const PtDevParams* __restrict__ dp, uint64_t P, uint32_t depth,
+__global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevParams* __restrict__ dp, uint64_t P, uint32_t depth,
                                                   const uint32_t* __restrict__ count_cur,
                                                   const cqx_ray* __restrict__ rays_cur,
                                                   const uint32_t* __restrict__ paths_cur,
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(256) k_pt_shade(const PtDevParams* __restrict_
 {
     const PtDevParams& d = *dp;
     const uint32_t n = *count_cur;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     bool cont = false;
     uint32_t p = 0;
     cqx_ray next;
@@ -235,18 +236,14 @@ __global__ void __launch_bounds__(256) k_pt_shade(const PtDevParams* __restrict_
             }
         }
     }
-    // compact the surviving rays: one atomic per wave, ballot prefix for the slots
-    const uint64_t mask = __ballot(cont);
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t base = 0;
-    if (mask) {
-        if (lane == uint32_t(__ffsll((long long)mask) - 1)) base = atomicAdd(count_next, uint32_t(__popcll(mask)));
-        base = __shfl(base, __ffsll((long long)mask) - 1);
-    }
+    // compact the surviving rays
+    const bool take[1] = { cont };
+    uint32_t* const counts[1] = { count_next };
+    uint32_t slot[1];
+    cqx::block_slots<cqx::kCompactBlock>(take, counts, slot);
     if (cont) {
-        const uint32_t j = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-        rays_next[j] = next;
-        paths_next[j] = p;
+        rays_next[slot[0]] = next;
+        paths_next[slot[0]] = p;
     }
 }
 
@@ -549,6 +546,7 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
     hipStream_t s = ctx->capture_stream;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const uint32_t gb = uint32_t((P + 255) / 256);
+    const uint32_t gc = uint32_t((P + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
     hipLaunchKernelGGL(k_pt_generate, dim3(gb), dim3(256), 0, s, p->dparams, P, p->rays[0], p->paths[0], p->beta, p->L,
                        p->rng, p->counts + 0);
     cqx_status cst = CQX_OK;
@@ -559,7 +557,7 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, cnt_next, 0u);
         cst = cqx::launch_intersect(ctx, p->vol, p->rays[cur], P, cnt_cur, CQX_SURFACE_PROPERTIES, p->max_footprint,
                                     p->hits, &p->cfg, s);
-        hipLaunchKernelGGL(k_pt_shade, dim3(gb), dim3(256), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
+        hipLaunchKernelGGL(k_pt_shade, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
                            p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next);
     }
     hipGraph_t g = nullptr;
