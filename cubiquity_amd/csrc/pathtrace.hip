@@ -20,8 +20,9 @@
 //              1024-thread block
 //   megakernel one lane per path, grid-stride
 // Both write per-path radiance; k_pt_resolve sums the samples of a pixel in sample order.
-// Adjoint (path replay): regenerates each path from its RNG (nothing is stored by the forward
-// pass), keeps the <= 8 vertices of the path in registers and sweeps them backwards,
+// Adjoint (path replay): replays the wavefront pass, which records one byte per path vertex (its
+// material) and how the path ended (the PCG streams make the replay the same paths), then a sweep
+// kernel rebuilds each path's throughputs in the forward's FP32 order and runs backwards,
 //   G_K = sky if escaped else 0;  G_j = E_j + A_j G_{j+1};
 //   dV/dE_j = beta_j,  dV/dA_j = beta_j G_{j+1},
 // scaled by dL/dpixel / spp; per-block partial sums in LDS, reduced over blocks in FP64.
@@ -39,6 +40,10 @@ using cqx::release;
 namespace {
 
 constexpr int kMaxVertices = 8; // adjoint: bounces + 1 <= 8
+// Path record of the wavefront pass, kept when bounces + 1 <= kMaxVertices: vmat[j * P + p] is
+// the material of path p's vertex j, vstat[p] its vertex count K, plus kEscaped when the path
+// ended in the sky. Paths that end by an anomaly or a failed scatter have kEscaped clear.
+constexpr uint32_t kEscaped = 0x80u;
 
 struct PtDevParams {
     float eye[3];
@@ -201,7 +206,8 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
                                                   const cqx_hit* __restrict__ hits, float* __restrict__ beta,
                                                   float* __restrict__ L, uint64_t* __restrict__ rng_state,
                                                   cqx_ray* __restrict__ rays_next, uint32_t* __restrict__ paths_next,
-                                                  uint32_t* __restrict__ count_next)
+                                                  uint32_t* __restrict__ count_next, uint8_t* __restrict__ vmat,
+                                                  uint8_t* __restrict__ vstat)
 {
     const PtDevParams& d = *dp;
     const uint32_t n = *count_cur;
@@ -215,10 +221,16 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
         float b[3] = { beta[p], beta[P + p], beta[2 * P + p] };
         if (h.info & CQX_HIT_ANOMALY) {
             // traversal gave up on this ray: the path ends without a contribution
+            if (vstat) vstat[p] = uint8_t(depth);
         } else if (!(h.info & CQX_HIT_FLAG)) {
             for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.sky[a];
+            if (vstat) vstat[p] = uint8_t(depth | kEscaped);
         } else {
             const uint32_t m = (h.info >> 16) & 0xffu;
+            if (vstat) {
+                vmat[depth * P + p] = uint8_t(m);
+                vstat[p] = uint8_t(depth + 1);
+            }
             for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.emission[m * 3 + a];
             if (depth < d.bounces) {
                 const cqx_ray r = rays_cur[i];
@@ -297,11 +309,14 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 
 // ---- adjoint (path replay) -----------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restrict__ dp, TraceParams T, uint64_t P,
+// The backward sweep over each path's recorded vertices. The throughputs beta_j are rebuilt as the
+// forward pass formed them (beta_0 = 1, beta_{j+1} = beta_j * albedo[m_j], FP32 in that order).
+__global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restrict__ dp, uint64_t P,
+                                                    const uint8_t* __restrict__ vmat,
+                                                    const uint8_t* __restrict__ vstat,
                                                     const float* __restrict__ dimage, uint32_t M,
                                                     float* __restrict__ partials)
 {
-    extern __shared__ uint32_t lds_stack[];
     __shared__ float g[256 * 6]; // [material][dA rgb, dE rgb]
     for (uint32_t k = threadIdx.x; k < 256 * 6; k += blockDim.x) g[k] = 0.0f;
     __syncthreads();
@@ -309,41 +324,25 @@ __global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restric
     const float inv_spp = 1.0f / float(d.spp);
     const uint64_t stride_all = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < P; p += stride_all) {
-        Pcg rng;
-        float o[3], dir[3];
-        uint32_t pixel;
-        path_start(d, p, rng, o, dir, pixel);
+        const uint32_t stat = vstat[p];
+        const int K = int(stat & ~kEscaped);
+        const bool escaped = (stat & kEscaped) != 0;
+        const uint32_t pixel = uint32_t(p) / d.spp; // P < 2^32
         float delta[3];
         for (int a = 0; a < 3; a++) delta[a] = dimage[uint64_t(pixel) * 3 + a] * inv_spp;
-        float b[3] = { 1.0f, 1.0f, 1.0f };
         uint32_t vm[kMaxVertices];
         float vb[kMaxVertices][3];
-        int K = 0;
-        bool escaped = false;
-        for (uint32_t depth = 0; depth < uint32_t(kMaxVertices); depth++) {
-            cqx_hit h;
-            trace_ray(T, o[0], o[1], o[2], dir[0], dir[1], dir[2], lds_stack + threadIdx.x, blockDim.x, h);
-            if (h.info & CQX_HIT_ANOMALY) break;
-            if (!(h.info & CQX_HIT_FLAG)) {
-                escaped = true;
-                break;
-            }
-            const uint32_t m = (h.info >> 16) & 0xffu;
+        float b[3] = { 1.0f, 1.0f, 1.0f };
 #pragma unroll
-            for (int j = 0; j < kMaxVertices; j++)
-                if (j == K) {
-                    vm[j] = m;
-                    vb[j][0] = b[0];
-                    vb[j][1] = b[1];
-                    vb[j][2] = b[2];
+        for (int j = 0; j < kMaxVertices; j++) {
+            if (j < K) {
+                const uint32_t m = vmat[uint64_t(j) * P + p];
+                vm[j] = m;
+                for (int a = 0; a < 3; a++) {
+                    vb[j][a] = b[a];
+                    b[a] = b[a] * d.albedo[m * 3 + a];
                 }
-            K++;
-            if (depth >= d.bounces) break;
-            float pos[3], nrm[3];
-            for (int a = 0; a < 3; a++) pos[a] = o[a] + dir[a] * h.t;
-            decode_normal(h.info, nrm);
-            if (!scatter(d, rng, pos, nrm, o, dir)) break;
-            for (int a = 0; a < 3; a++) b[a] = b[a] * d.albedo[m * 3 + a];
+            }
         }
         float G[3];
         for (int a = 0; a < 3; a++) G[a] = escaped ? d.sky[a] : 0.0f;
@@ -405,6 +404,8 @@ struct PtPlan {
     float* L = nullptr;
     uint64_t* rng = nullptr;
     uint32_t* counts = nullptr;
+    uint8_t* vmat = nullptr;  // (bounces + 1) x P vertex materials, when bounces + 1 <= kMaxVertices
+    uint8_t* vstat = nullptr; // P path records (vertex count, escaped)
     float* partials = nullptr;
     size_t partial_floats = 0;
     hipGraph_t graph = nullptr;
@@ -426,6 +427,8 @@ void destroy_pt_plan(PtPlan* p)
     release(p->L);
     release(p->rng);
     release(p->counts);
+    release(p->vmat);
+    release(p->vstat);
     release(p->partials);
     delete p;
 }
@@ -542,6 +545,10 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
     if (st == CQX_OK) st = alloc(&p->beta, 3 * P);
     if (st == CQX_OK) st = alloc(&p->rng, P);
     if (st == CQX_OK) st = alloc(&p->counts, 64);
+    if (st == CQX_OK && p->bounces + 1 <= uint32_t(kMaxVertices)) { // the adjoint's path record
+        st = alloc(&p->vmat, (p->bounces + 1) * P);
+        if (st == CQX_OK) st = alloc(&p->vstat, P);
+    }
     if (st != CQX_OK) return st;
     hipStream_t s = ctx->capture_stream;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -558,7 +565,8 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
         cst = cqx::launch_intersect(ctx, p->vol, p->rays[cur], P, cnt_cur, CQX_SURFACE_PROPERTIES, p->max_footprint,
                                     p->hits, &p->cfg, s);
         hipLaunchKernelGGL(k_pt_shade, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
-                           p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next);
+                           p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next,
+                           p->vmat, p->vstat);
     }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
@@ -650,13 +658,11 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     if (st != CQX_OK) return st;
     PtPlan* p = nullptr;
     st = get_plan(ctx, vol, pp, cfg, &p);
+    if (st == CQX_OK) st = ensure_wavefront(ctx, p);
     if (st != CQX_OK) return st;
     const uint32_t block = 256;
-    const size_t lds = size_t(vol->max_height + 1) * block * sizeof(uint32_t);
-    if (lds + 256 * 6 * sizeof(float) > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "not enough LDS");
-    const uint32_t waves = (cfg && cfg->waves_per_cu) ? cfg->waves_per_cu : 16u;
-    const uint64_t want = (uint64_t(ctx->cu_count) * waves * 64 + block - 1) / block;
-    const uint32_t blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(want, (p->P + block - 1) / block)));
+    const uint32_t blocks =
+        uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(ctx->cu_count) * 8, (p->P + block - 1) / block)));
     const size_t need = size_t(blocks) * pp->n_materials * 6;
     if (need > p->partial_floats) {
         HIP_TRY(hipDeviceSynchronize());
@@ -669,8 +675,10 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     }
     hipStream_t s = hipStream_t(stream_ptr);
     HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), lds, s, p->dparams, trace_params(vol, pp, cfg), p->P,
-                       d_image, pp->n_materials, p->partials);
+    // replay the paths (the wavefront graph records each path's vertices), then sweep them backwards
+    HIP_TRY(hipGraphLaunch(p->exec, s));
+    hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat, d_image,
+                       pp->n_materials, p->partials);
     hipLaunchKernelGGL(k_pt_reduce, dim3(pp->n_materials * 6), dim3(256), 0, s, p->partials, blocks,
                        pp->n_materials, d_albedo, d_emission);
     HIP_TRY(hipGetLastError());

@@ -385,9 +385,9 @@ typedef struct {
 cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params, uint32_t mode,
                          const cqx_launch_cfg* cfg, float* image, void* stream);
 
-/* Gradient of a scalar image loss with respect to the material tables, by path replay: every
- * path of cqx_pathtrace is regenerated from its RNG (nothing is stored by the forward pass),
- * geometry and traversal are discrete and exact. d_image = dLoss/dimage (device, height x width
+/* Gradient of a scalar image loss with respect to the material tables, by path replay: the
+ * paths of cqx_pathtrace are traced again (same RNG streams, wavefront form) with each vertex's
+ * material recorded, then swept backwards; geometry and traversal are discrete and exact. d_image = dLoss/dimage (device, height x width
  * x 3); d_albedo, d_emission (device, n_materials x 3) receive dLoss/dalbedo and
  * dLoss/demission. At most 7 bounces. Synchronizes `stream` before returning. */
 cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params,
