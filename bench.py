@@ -328,6 +328,44 @@ def bench_large(cq, W, ctx, frames, w=None):
             "launch": {f2: getattr(best, f2) for f2, _ in best._fields_}}
 
 
+def bench_frame_launches(cq, W, ctx, reps):
+    """Frame time of the reference-shading frame and of the wavefront path tracer (1024^3,
+    1280x720; 16 spp, 4 bounces) under several traversal launches, against their "auto" launch.
+    Every image must equal the auto launch's bit for bit."""
+    import torch
+
+    w = W.PATHTRACE_1024
+    hv = cq.HostVolume.generate(w.params())
+    dv = ctx.upload(hv)
+    alb, em = cq.material_table(9, 11)
+    rays = torch.from_numpy(cq.camera_rays(w.camera.eye, w.camera.target, w.camera.fov_y, 1280, 720)).cuda()
+    colours = np.zeros((256, 3), np.float32)
+    colours[1:10] = alb
+    setup = cq.PathTraceSetup(w.camera, 1280, 720, 16, 4, alb, em)
+    cfgs = {"auto": None}
+    for b in (64, 128, 256):
+        cfgs[f"lane_b{b}"] = cq.LaunchCfg(variant=1, block=b)
+    for wpc in (8, 16, 24, 32):
+        cfgs[f"persistent_w{wpc}"] = cq.LaunchCfg(variant=2, waves_per_cu=wpc, regroup_interval=8, refill_lanes=16)
+    cfgs["node_cache_w32"] = cq.LaunchCfg(variant=3, block=128, waves_per_cu=32, regroup_interval=8, refill_lanes=16)
+    out = {"demo_frame_ms": {}, "pathtrace_frame_ms": {}}
+    img = torch.empty((rays.shape[0], 3), dtype=torch.float32, device="cuda")
+    ref = cq.demo_trace(ctx, dv, rays, colours, seed=1).clone()
+    pref = cq.pathtrace(ctx, dv, setup, 0).clone()
+    for k, c in cfgs.items():
+        ms = timed(lambda: cq.demo_trace(ctx, dv, rays, colours, seed=1, out=img, cfg=c), reps)
+        if not torch.equal(img, ref):
+            raise SystemExit(f"frame launches: reference-shading image differs under {k}")
+        out["demo_frame_ms"][k] = round(ms, 3)
+        ms = timed(lambda: cq.pathtrace(ctx, dv, setup, 0, cfg=c), max(2, reps // 2))
+        if not torch.equal(cq.pathtrace(ctx, dv, setup, 0, cfg=c), pref):
+            raise SystemExit(f"frame launches: path-traced image differs under {k}")
+        out["pathtrace_frame_ms"][k] = round(ms, 3)
+        log(f"frame launches: {k}: demo {out['demo_frame_ms'][k]} ms, path tracing {out['pathtrace_frame_ms'][k]} ms")
+    dv.free()
+    return out
+
+
 def bench_scale(cq, W, ctx, reps):
     """The traversal on larger volumes of the headline family (2^24 incoherent rays each, tuned
     launch): node store size, Mrays/s and algorithmic bandwidth, from MALL- to HBM-resident."""
@@ -408,6 +446,8 @@ def main():
                     help="also run the full-size edit workload (16384^3, 1M spheres; needs ~70 GB of host memory)")
     ap.add_argument("--scale", action="store_true",
                     help="also time the traversal on the 4096^3 and 8192^3 volumes of the same family")
+    ap.add_argument("--frame-launches", action="store_true",
+                    help="also time the reference-shading and path-tracing frames under several traversal launches")
     ap.add_argument("--log2-size", type=int, default=10,
                     help="side of the traversal volume (for profiling at other sizes; the metric is quoted at 10)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch defaults)")
@@ -543,6 +583,8 @@ def main():
     if args.large:
         out["large_16384"] = bench_large(cq, W, ctx, 3)
         log(f"large: {out['large_16384']}")
+    if args.frame_launches:
+        out["frame_launches"] = bench_frame_launches(cq, W, ctx, max(2, args.steps // 2))
     if args.scale:
         out["scale"] = bench_scale(cq, W, ctx, args.steps)
         log(f"scale: {out['scale']}")

@@ -6,8 +6,9 @@
 // threshold (idle lanes that trigger a refill). The binning prepass is inside the timed span. Each
 // candidate is timed with HIP events (median of 3 after a warm-up); the search is coordinate
 // descent from the built-in defaults, two sweeps. Results are cached in the context per
-// (SubDAG height, batch-size class = floor(log2 n)) and used by every launch whose configuration
-// is left at "auto" (cfg == NULL or cfg->variant == 0).
+// (SubDAG height, batch-size class = floor(log2 n)) and used by every cqx_intersect whose
+// configuration is left at "auto" (cfg == NULL or cfg->variant == 0), also for volumes and batches
+// within 2 of a tuned height and class (tuned_cfg).
 //
 // The traversal stack depth is not a free parameter: it is max SubDAG height + 1 LDS rows, the
 // least that holds every node the walk can pop back to. A shorter stack would need restarts that
@@ -30,15 +31,21 @@ int batch_class(uint64_t n)
     return c;
 }
 
+// Entries within 2 of the SubDAG height and of the batch-size class qualify; the nearest height
+// wins, then the nearest class. Heights other than the exact one matter after edits: an edit
+// that reaches past the volume's bounds starts the SubDAG one level higher (and adds SubDAGs in
+// other octants), and the launch tuned on the volume before the edit suits it far better than
+// the built-in defaults (at 16384^3: 34 against 60 ms per 2^26 rays).
 const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n)
 {
     const int cls = batch_class(n);
     const cqx_ctx::Tuned* best = nullptr;
     int best_d = 1 << 30;
     for (const auto& t : ctx->tuned) {
-        if (t.height != height) continue;
-        const int d = std::abs(t.cls - cls);
-        if (d < best_d && d <= 2) {
+        const int dh = std::abs(t.height - height), dc = std::abs(t.cls - cls);
+        if (dh > 2 || dc > 2) continue;
+        const int d = dh * 8 + dc;
+        if (d < best_d) {
             best_d = d;
             best = &t;
         }

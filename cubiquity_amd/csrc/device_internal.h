@@ -65,7 +65,7 @@ struct cqx_dvolume {
 namespace cqx {
 
 // Autotuner cache lookup: the tuned configuration for a volume of this SubDAG height and a batch of
-// about n rays, or null.
+// about n rays (or of the nearest tuned height and batch size, each within 2), or null.
 const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n);
 
 // Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).

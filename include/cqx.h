@@ -315,8 +315,11 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
 /* Launch autotuner (no reference counterpart). Times kernel variant, rays per workgroup,
  * persistent waves per CU, regroup interval and refill threshold on these rays (device pointers;
  * `hits` is overwritten), picks the fastest by coordinate descent and caches it in the context
- * for this volume depth and batch-size class; later launches with cfg == NULL or variant 0 use
- * it. Optional outputs: the chosen configuration and its median time in ms. */
+ * for this volume depth and batch-size class; later cqx_intersect calls with cfg == NULL or
+ * variant 0 use it, also on volumes and batches within 2 of the tuned depth and class (an edit
+ * past the volume's bounds deepens its SubDAG by one). cqx_pathtrace and cqx_demo_trace have
+ * their own "auto" launches. Optional outputs: the chosen configuration and its median time in
+ * ms. */
 cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                         uint32_t flags, float max_footprint, cqx_hit* hits, void* stream,
                         cqx_launch_cfg* best, float* best_ms);
@@ -381,7 +384,8 @@ typedef struct {
 
 /* mode 0: wavefront (traversal kernel per bounce, captured in a hipGraph, replayed per call);
  * mode 1: megakernel (one lane per path). Both give the same image bit for bit. `image` is a
- * device pointer; synchronizes `stream` before returning. */
+ * device pointer; synchronizes `stream` before returning. cfg NULL or variant 0: the bounce
+ * launches use persistent waves, 24 per CU unless cfg sets waves_per_cu. */
 cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params, uint32_t mode,
                          const cqx_launch_cfg* cfg, float* image, void* stream);
 
@@ -411,6 +415,7 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
  * anomalies count as misses.
  * rays: device, n x 6 floats (the camera rays of PathtracingDemo::raytrace, :214-225); out_rgb:
  * device, n x 3 floats, overwritten or, with accumulate, added to (the demo's mImage +=).
+ * cfg NULL or variant 0: one ray per lane, 128 per workgroup, for every traversal of the frame.
  * Asynchronous on `stream`; one call in flight per context. n <= 2^30. */
 typedef struct {
     const float* colours; /* host, 256 x 3: the material colours (cqx_materials_colour_table) */
