@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -561,6 +562,11 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
             hv->v.root = root;
             return CQX_OK;
         }
+        using clock = std::chrono::steady_clock;
+        const auto ms_since = [](clock::time_point t) {
+            return std::chrono::duration<double, std::milli>(clock::now() - t).count();
+        };
+        const clock::time_point t_start = clock::now();
         // Regions each edit's bounds touch (one voxel of margin), edits kept in order per region.
         std::vector<std::pair<RegionKey, uint32_t>> touch;
         for (uint64_t i = 0; i < n; i++) {
@@ -613,17 +619,27 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
                 next = nr;
             }
         };
+        const double plan_ms = ms_since(t_start);
+        const clock::time_point t_apply = clock::now();
         const int nt = int(std::min<size_t>(size_t(threads), nr));
         std::vector<std::thread> pool;
         for (int t = 1; t < nt; t++) pool.emplace_back(work);
         work();
         for (std::thread& t : pool) t.join();
         if (oom) throw std::bad_alloc();
+        const double apply_ms = ms_since(t_apply);
+        const clock::time_point t_merge = clock::now();
         merge_regions(nodes, in, base, stores, results, threads);
         stores.clear();
+        const double merge_ms = ms_since(t_merge);
+        const clock::time_point t_top = clock::now();
         std::vector<uint32_t> all(nr);
         for (size_t r = 0; r < nr; r++) all[r] = uint32_t(r);
         hv->v.root = rebuild_top(g, hv->v.root, kRootHeight, hs, keys, results, all);
+        log_debug("edits: %llu spheres over %zu regions of side 2^%d on %d threads; plan %.1f ms, apply %.1f ms, "
+                  "merge %.1f ms, top levels %.1f ms; %zu nodes appended",
+                  (unsigned long long)n, nr, hs, nt, plan_ms, apply_ms, merge_ms, ms_since(t_top),
+                  nodes.size() - size_t(base));
         return CQX_OK;
     } catch (const std::bad_alloc&) {
         return fail(CQX_ERR_OOM, "out of memory while editing");
