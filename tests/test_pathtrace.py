@@ -180,3 +180,40 @@ def test_gpu_adjoint_vs_fp64_and_fd(cq, oracle, gpu_ctx, scene16):
     assert per.max() <= 1e-3 and l2 <= 1e-3, (per.max(), l2)
     dv.free()
     torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bounces", [0, 7])
+def test_gpu_adjoint_at_bounce_limits(cq, oracle, gpu_ctx, scene16, bounces):
+    """The adjoint replays the wavefront pass, which records up to bounces + 1 vertex materials
+    per path, then sweeps the record: at 0 bounces (one vertex) and at 7 (eight, the most the
+    record holds) the gradient must equal the FP64 one over the same paths."""
+    import torch
+
+    hv, s0, _ = scene16
+    s = cq.PathTraceSetup(W._camera(6, 32, 32), 32, 32, 2, bounces, s0.albedo, s0.emission)
+    dv = gpu_ctx.upload(hv)
+    img = cq.pathtrace(gpu_ctx, dv, s, 0).double()
+    d_image = (2 * img).float()  # dLoss/dimage of sum(image^2)
+    da, de = cq.pathtrace_adjoint(gpu_ctx, dv, s, d_image)
+    da, de = da.cpu().numpy().astype(np.float64), de.cpu().numpy().astype(np.float64)
+    _, ra, re = oracle.pathtrace_f64(hv.nodes, hv.root, s.params, d_image.cpu().numpy().astype(np.float64))
+    g = np.concatenate([da.ravel(), de.ravel()])
+    r = np.concatenate([ra.ravel(), re.ravel()])
+    assert np.abs(r).max() > 0
+    scale = np.maximum(np.abs(r), 1e-3 * np.abs(r).max())
+    assert (np.abs(g - r) <= 1e-4 * scale).all(), (bounces, np.abs(g - r).max())
+    dv.free()
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("spp", [64, 65])
+def test_gpu_resolve_both_paths(cq, oracle, gpu_ctx, scene, dscene, spp):
+    """k_pt_resolve stages a block's samples in LDS up to 64 spp and reads them directly above;
+    both sum each pixel's samples in sample order, so the image equals the oracle's bit for bit."""
+    hv, s0 = scene
+    s = cq.PathTraceSetup(W.RAYCAST_128.camera, 12, 8, spp, 1, s0.albedo, s0.emission)
+    img = cq.pathtrace(gpu_ctx, dscene, s, 0).cpu().numpy()
+    ref = oracle.pathtrace(hv.nodes, hv.root, s.params)
+    assert np.array_equal(img, ref), f"spp {spp}: {(img != ref).sum()} values differ"
