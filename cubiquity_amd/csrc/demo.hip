@@ -27,10 +27,12 @@
 // (0, 0, 0): the reference loops forever on a state cycle that never lands inside the sphere.
 // Rays the traversal flags as anomalies (the reference would not return) count as misses.
 //
-// Wavefront: trace primaries -> k_demo_primary (sun/sky shadow rays and the reflected ray, each
-// list compacted by one atomic per 1024-thread block) -> trace shadow rays -> k_demo_visibility -> trace
-// reflected rays -> k_demo_bounce (second-hit shadow rays) -> trace -> k_demo_visibility ->
-// k_demo_resolve. Ray counts stay on the device; the persistent traversal kernel reads them.
+// Wavefront: trace primaries -> k_demo_primary (the sun and sky shadow rays and the reflected ray
+// of each hit, in one list compacted with one atomic per list per 1024-thread block) -> trace
+// that list -> k_demo_bounce (first visibilities; second hits and their shadow rays) -> trace ->
+// k_demo_visibility -> k_demo_resolve. The first shadow rays and the reflected rays are
+// independent, so one traversal launch serves both. Ray counts stay on the device; the traversal
+// kernels read them.
 // Visibility is kept per (pixel, shadow ray) and summed in the reference's order at the end.
 #include "device_internal.h"
 
@@ -46,8 +48,10 @@ using cqx::release;
 namespace {
 
 constexpr int kMaxDraws = 1024;
-// shadow-ray kinds: a shadow ray's tag is pixel x 4 + kind, also its visibility byte
+// shadow-ray kinds: a shadow ray's tag is pixel x 4 + kind, also its visibility byte. In the
+// first-hit list (where second shadow rays never occur) kind kRefl marks a reflected ray.
 constexpr uint32_t kSun0 = 0, kSky0 = 1, kSun1 = 2, kSky1 = 3;
+constexpr uint32_t kRefl = 2;
 
 struct DemoDev {
     const float* colours; // 256 x 3, device
@@ -208,10 +212,8 @@ __global__ void k_demo_zero(uint32_t* __restrict__ counts)
 __global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_primary(DemoDev d, uint64_t n, const cqx_ray* __restrict__ rays,
                                                       const cqx_hit* __restrict__ hits, uint32_t* __restrict__ state,
                                                       uint32_t* __restrict__ info0, float* __restrict__ surf0,
-                                                      uint32_t* __restrict__ vis, cqx_ray* __restrict__ sh_rays,
-                                                      uint32_t* __restrict__ sh_tags, uint32_t* __restrict__ sh_count,
-                                                      cqx_ray* __restrict__ rf_rays, uint32_t* __restrict__ rf_tags,
-                                                      uint32_t* __restrict__ rf_count)
+                                                      uint32_t* __restrict__ vis, cqx_ray* __restrict__ sec_rays,
+                                                      uint32_t* __restrict__ sec_tags, uint32_t* __restrict__ sec_count)
 {
     __shared__ PointTable pts;
     fill_point_table(pts);
@@ -244,21 +246,17 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_primary(DemoDev d, 
         }
     }
     const bool take[3] = { hit && d.include_sun, hit && d.include_sky, hit };
-    uint32_t* const counts[3] = { sh_count, sh_count, rf_count };
+    uint32_t* const counts[3] = { sec_count, sec_count, sec_count };
     uint32_t k[3];
     cqx::block_slots<cqx::kCompactBlock>(take, counts, k);
-    if (take[0]) {
-        sh_rays[k[0]] = sun;
-        sh_tags[k[0]] = uint32_t(p) * 4u + kSun0;
-    }
-    if (take[1]) {
-        sh_rays[k[1]] = sky;
-        sh_tags[k[1]] = uint32_t(p) * 4u + kSky0;
-    }
-    if (take[2]) {
-        rf_rays[k[2]] = refl;
-        rf_tags[k[2]] = uint32_t(p);
-    }
+    const cqx_ray r[3] = { sun, sky, refl };
+    const uint32_t kind[3] = { kSun0, kSky0, kRefl };
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+        if (take[l]) {
+            sec_rays[k[l]] = r[l];
+            sec_tags[k[l]] = uint32_t(p) * 4u + kind[l];
+        }
 }
 
 // A shadow ray that hits nothing lets its light through.
@@ -272,34 +270,40 @@ __global__ void __launch_bounds__(256) k_demo_visibility(const uint32_t* __restr
     vis[tags[i]] = (hits[i].info & CQX_HIT_FLAG) ? 0u : 1u;
 }
 
-// Second hits (the reflected rays): surface colour and the second gatherLighting's shadow rays.
-__global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_bounce(DemoDev d, uint64_t cap, const uint32_t* __restrict__ rf_count,
-                                                     const cqx_ray* __restrict__ rf_rays,
-                                                     const uint32_t* __restrict__ rf_tags,
-                                                     const cqx_hit* __restrict__ rf_hits, const uint32_t* __restrict__ state,
-                                                     uint32_t* __restrict__ info1, float* __restrict__ surf1,
-                                                     cqx_ray* __restrict__ sh_rays, uint32_t* __restrict__ sh_tags,
-                                                     uint32_t* __restrict__ sh_count)
+// The traced first-hit list: a shadow ray's visibility; for a reflected ray, the second hit's
+// surface colour and the second gatherLighting's shadow rays.
+__global__ void __launch_bounds__(cqx::kCompactBlock) k_demo_bounce(DemoDev d, uint64_t cap, const uint32_t* __restrict__ sec_count,
+                                                     const cqx_ray* __restrict__ sec_rays,
+                                                     const uint32_t* __restrict__ sec_tags,
+                                                     const cqx_hit* __restrict__ sec_hits, const uint32_t* __restrict__ state,
+                                                     uint8_t* __restrict__ vis, uint32_t* __restrict__ info1,
+                                                     float* __restrict__ surf1, cqx_ray* __restrict__ sh_rays,
+                                                     uint32_t* __restrict__ sh_tags, uint32_t* __restrict__ sh_count)
 {
     __shared__ PointTable pts;
     fill_point_table(pts);
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint64_t c = *rf_count;
+    const uint64_t c = *sec_count;
     bool hit = false;
     uint32_t p = 0;
     cqx_ray sun, sky;
     if (i < c && i < cap) {
-        p = rf_tags[i];
-        const cqx_hit h = rf_hits[i];
-        info1[p] = h.info;
-        hit = (h.info & CQX_HIT_FLAG) != 0;
-        if (hit) {
-            float pos[3], nrm[3], col[3];
-            hit_surface(d, rf_rays[i], h, pos, nrm, col);
+        const uint32_t tag = sec_tags[i];
+        const cqx_hit h = sec_hits[i];
+        p = tag >> 2;
+        if ((tag & 3u) != kRefl) {
+            vis[tag] = (h.info & CQX_HIT_FLAG) ? 0u : 1u; // a shadow ray that hits nothing lets its light through
+        } else {
+            info1[p] = h.info;
+            hit = (h.info & CQX_HIT_FLAG) != 0;
+            if (hit) {
+                float pos[3], nrm[3], col[3];
+                hit_surface(d, sec_rays[i], h, pos, nrm, col);
 #pragma unroll
-            for (int a = 0; a < 3; a++) surf1[uint64_t(p) * 3 + a] = col[a];
-            uint32_t s = state[p];
-            shadow_rays(d, pos, nrm, s, pts, sun, sky);
+                for (int a = 0; a < 3; a++) surf1[uint64_t(p) * 3 + a] = col[a];
+                uint32_t s = state[p];
+                shadow_rays(d, pos, nrm, s, pts, sun, sky);
+            }
         }
     }
     const bool take[2] = { hit && d.include_sun, hit && d.include_sky };
@@ -383,12 +387,12 @@ struct DemoPlan {
     float* surf0 = nullptr;
     float* surf1 = nullptr;
     uint32_t* vis = nullptr; // 4 bytes per pixel
-    cqx_ray* sh_rays = nullptr;
+    cqx_ray* sec_rays = nullptr; // first-hit list: sun and sky shadow rays, reflected rays (3 per pixel)
+    uint32_t* sec_tags = nullptr;
+    cqx_hit* sec_hits = nullptr;
+    cqx_ray* sh_rays = nullptr; // second-hit shadow rays (2 per pixel)
     uint32_t* sh_tags = nullptr;
     cqx_hit* sh_hits = nullptr;
-    cqx_ray* rf_rays = nullptr;
-    uint32_t* rf_tags = nullptr;
-    cqx_hit* rf_hits = nullptr;
 };
 
 void destroy_demo_plan(DemoPlan* p)
@@ -425,8 +429,8 @@ cqx_status demo_plan(cqx_ctx* ctx, uint64_t n, DemoPlan** out)
     const size_t o_col = take(256 * 3 * sizeof(float)), o_cnt = take(4 * sizeof(uint32_t));
     const size_t o_h0 = take(n * sizeof(cqx_hit)), o_st = take(n * 4), o_i0 = take(n * 4), o_i1 = take(n * 4);
     const size_t o_s0 = take(n * 12), o_s1 = take(n * 12), o_vis = take(n * 4);
+    const size_t o_ser = take(3 * n * sizeof(cqx_ray)), o_set = take(3 * n * 4), o_seh = take(3 * n * sizeof(cqx_hit));
     const size_t o_shr = take(2 * n * sizeof(cqx_ray)), o_sht = take(2 * n * 4), o_shh = take(2 * n * sizeof(cqx_hit));
-    const size_t o_rfr = take(n * sizeof(cqx_ray)), o_rft = take(n * 4), o_rfh = take(n * sizeof(cqx_hit));
     if (hipMalloc(&p->mem, off) != hipSuccess) {
         delete p;
         return fail(CQX_ERR_OOM, "device allocation of %zu bytes for %llu pixels failed", off, (unsigned long long)n);
@@ -442,12 +446,12 @@ cqx_status demo_plan(cqx_ctx* ctx, uint64_t n, DemoPlan** out)
     p->surf0 = reinterpret_cast<float*>(b + o_s0);
     p->surf1 = reinterpret_cast<float*>(b + o_s1);
     p->vis = reinterpret_cast<uint32_t*>(b + o_vis);
+    p->sec_rays = reinterpret_cast<cqx_ray*>(b + o_ser);
+    p->sec_tags = reinterpret_cast<uint32_t*>(b + o_set);
+    p->sec_hits = reinterpret_cast<cqx_hit*>(b + o_seh);
     p->sh_rays = reinterpret_cast<cqx_ray*>(b + o_shr);
     p->sh_tags = reinterpret_cast<uint32_t*>(b + o_sht);
     p->sh_hits = reinterpret_cast<cqx_hit*>(b + o_shh);
-    p->rf_rays = reinterpret_cast<cqx_ray*>(b + o_rfr);
-    p->rf_tags = reinterpret_cast<uint32_t*>(b + o_rft);
-    p->rf_hits = reinterpret_cast<cqx_hit*>(b + o_rfh);
     ctx->demo_plan = p;
     *out = p;
     return CQX_OK;
@@ -504,24 +508,23 @@ cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* r
     c2.bin_bits = 0;
     const uint32_t g1 = uint32_t((n + 255) / 256), g2 = uint32_t((2 * n + 255) / 256);
     const uint32_t gc = uint32_t((n + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
-    uint32_t* cnt = p->counts; // [0] first shadow rays, [1] reflected rays, [2] second shadow rays
+    const uint32_t gc3 = uint32_t((3 * n + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
+    uint32_t* cnt = p->counts; // [0] first-hit list, [1] second shadow rays
+    uint8_t* vis = reinterpret_cast<uint8_t*>(p->vis);
     hipLaunchKernelGGL(k_demo_zero, dim3(1), dim3(64), 0, s, cnt);
     st = cqx::launch_intersect(ctx, vol, rays, n, nullptr, CQX_SURFACE_PROPERTIES, fp, p->hits0, &c1, s);
     if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_primary, dim3(gc), dim3(cqx::kCompactBlock), 0, s, d, n, rays, p->hits0, p->state, p->info0, p->surf0,
-                       p->vis, p->sh_rays, p->sh_tags, cnt + 0, p->rf_rays, p->rf_tags, cnt + 1);
-    st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 0, 0u, fp, p->sh_hits, &c2, s);
+    hipLaunchKernelGGL(k_demo_primary, dim3(gc), dim3(cqx::kCompactBlock), 0, s, d, n, rays, p->hits0, p->state,
+                       p->info0, p->surf0, p->vis, p->sec_rays, p->sec_tags, cnt + 0);
+    // Surface properties are for the reflected rays; a shadow ray reads only the hit flag, which
+    // does not depend on them.
+    st = cqx::launch_intersect(ctx, vol, p->sec_rays, 3 * n, cnt + 0, CQX_SURFACE_PROPERTIES, fp, p->sec_hits, &c2, s);
     if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_visibility, dim3(g2), dim3(256), 0, s, cnt + 0, 2 * n, p->sh_tags, p->sh_hits,
-                       reinterpret_cast<uint8_t*>(p->vis));
-    st = cqx::launch_intersect(ctx, vol, p->rf_rays, n, cnt + 1, CQX_SURFACE_PROPERTIES, fp, p->rf_hits, &c2, s);
+    hipLaunchKernelGGL(k_demo_bounce, dim3(gc3), dim3(cqx::kCompactBlock), 0, s, d, 3 * n, cnt + 0, p->sec_rays,
+                       p->sec_tags, p->sec_hits, p->state, vis, p->info1, p->surf1, p->sh_rays, p->sh_tags, cnt + 1);
+    st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 1, 0u, fp, p->sh_hits, &c2, s);
     if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_bounce, dim3(gc), dim3(cqx::kCompactBlock), 0, s, d, n, cnt + 1, p->rf_rays, p->rf_tags, p->rf_hits,
-                       p->state, p->info1, p->surf1, p->sh_rays, p->sh_tags, cnt + 2);
-    st = cqx::launch_intersect(ctx, vol, p->sh_rays, 2 * n, cnt + 2, 0u, fp, p->sh_hits, &c2, s);
-    if (st != CQX_OK) return st;
-    hipLaunchKernelGGL(k_demo_visibility, dim3(g2), dim3(256), 0, s, cnt + 2, 2 * n, p->sh_tags, p->sh_hits,
-                       reinterpret_cast<uint8_t*>(p->vis));
+    hipLaunchKernelGGL(k_demo_visibility, dim3(g2), dim3(256), 0, s, cnt + 1, 2 * n, p->sh_tags, p->sh_hits, vis);
     hipLaunchKernelGGL(k_demo_resolve, dim3(g1), dim3(256), 0, s, d, n, p->info0, p->info1, p->surf0, p->surf1,
                        reinterpret_cast<const uint8_t*>(p->vis), out_rgb);
     HIP_TRY(hipGetLastError());
