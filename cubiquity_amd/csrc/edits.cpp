@@ -291,24 +291,34 @@ struct EditWalk {
     uint32_t fill(uint32_t node, int height, int32_t lx, int32_t ly, int32_t lz)
     {
         const uint32_t ch = uint32_t(height - 1);
+        const uint32_t side = 1u << ch;
+        // overlaps(brush.bounds(), childBounds), geometry.h:346-357, decided per axis: a child's
+        // extent along an axis depends only on its position (0 or 1) along that axis, so two
+        // tests per axis give the overlapping children (bit p of ok[a]: position p on axis a).
+        const int32_t lo[3] = { lx, ly, lz };
+        float f0[3][2], f1[3][2];
+        uint32_t ok[3] = { 0, 0, 0 };
+        for (int a = 0; a < 3; a++)
+            for (uint32_t p = 0; p <= 1; p++) {
+                const int32_t v0 = int32_t(uint32_t(lo[a]) + side * p);
+                const int32_t v1 = int32_t(uint32_t(v0) + (side - 1));
+                f0[a][p] = float(v0);
+                f1[a][p] = float(v1);
+                if (!(b.hi[a] < f0[a][p] || b.lo[a] > f1[a][p])) ok[a] |= 1u << p;
+            }
+        if (!(ok[0] && ok[1] && ok[2])) return node; // no child overlaps: nothing changes
         Node out;
         for (int c = 0; c < 8; c++) out.c[c] = st.child(node, uint32_t(c));
         bool changed = false;
         for (uint32_t cz = 0; cz <= 1; cz++)
             for (uint32_t cy = 0; cy <= 1; cy++)
                 for (uint32_t cx = 0; cx <= 1; cx++) {
+                    if (!((ok[0] >> cx) & (ok[1] >> cy) & (ok[2] >> cz) & 1u)) continue;
                     const uint32_t id = cz << 2 | cy << 1 | cx;
-                    const uint32_t side = 1u << ch;
                     const int32_t x0 = int32_t(uint32_t(lx) + side * cx), y0 = int32_t(uint32_t(ly) + side * cy),
                                   z0 = int32_t(uint32_t(lz) + side * cz);
-                    const int32_t x1 = int32_t(uint32_t(x0) + (side - 1)), y1 = int32_t(uint32_t(y0) + (side - 1)),
-                                  z1 = int32_t(uint32_t(z0) + (side - 1));
-                    const float fx0 = float(x0), fy0 = float(y0), fz0 = float(z0);
-                    const float fx1 = float(x1), fy1 = float(y1), fz1 = float(z1);
-                    // overlaps(brush.bounds(), childBounds), geometry.h:346-357
-                    if (b.hi[0] < fx0 || b.lo[0] > fx1 || b.hi[1] < fy0 || b.lo[1] > fy1 || b.hi[2] < fz0 ||
-                        b.lo[2] > fz1)
-                        continue;
+                    const float fx0 = f0[0][cx], fy0 = f0[1][cy], fz0 = f0[2][cz];
+                    const float fx1 = f1[0][cx], fy1 = f1[1][cy], fz1 = f1[2][cz];
                     const uint32_t child = out.c[id];
                     if (child == mat) continue;
                     // a child box wholly outside the sphere would come back unchanged from fill()
