@@ -170,8 +170,9 @@ __device__ __forceinline__ void path_start(const PtDevParams& d, uint64_t p, Pcg
 
 // ---- wavefront kernels -----------------------------------------------------------------------
 
+// Camera rays and generator states of all paths. The paths' throughput (1) and radiance (0) are
+// not stored: the shading pass at depth 0, which sees every path, starts from those values.
 __global__ void k_pt_generate(const PtDevParams* __restrict__ dp, uint64_t P, cqx_ray* __restrict__ rays,
-                              uint32_t* __restrict__ paths, float* __restrict__ beta, float* __restrict__ L,
                               uint64_t* __restrict__ rng_state, uint32_t* __restrict__ count)
 {
     const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -186,11 +187,8 @@ __global__ void k_pt_generate(const PtDevParams* __restrict__ dp, uint64_t P, cq
     for (int a = 0; a < 3; a++) {
         r.o[a] = o[a];
         r.d[a] = dir[a];
-        beta[a * P + p] = 1.0f;
-        L[a * P + p] = 0.0f;
     }
     rays[p] = r;
-    paths[p] = uint32_t(p);
     rng_state[p] = rng.state;
 }
 
@@ -216,14 +214,20 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
     uint32_t p = 0;
     cqx_ray next;
     if (i < n) {
-        p = paths_cur[i];
+        // depth 0 holds every path in order, with throughput 1 and radiance 0
+        p = depth == 0 ? uint32_t(i) : paths_cur[i];
         const cqx_hit h = hits[i];
-        float b[3] = { beta[p], beta[P + p], beta[2 * P + p] };
+        float b[3] = { 1.0f, 1.0f, 1.0f }, l[3] = { 0.0f, 0.0f, 0.0f };
+        if (depth > 0)
+            for (int a = 0; a < 3; a++) {
+                b[a] = beta[a * P + p];
+                l[a] = L[a * P + p];
+            }
         if (h.info & CQX_HIT_ANOMALY) {
             // traversal gave up on this ray: the path ends without a contribution
             if (vstat) vstat[p] = uint8_t(depth);
         } else if (!(h.info & CQX_HIT_FLAG)) {
-            for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.sky[a];
+            for (int a = 0; a < 3; a++) l[a] += b[a] * d.sky[a];
             if (vstat) vstat[p] = uint8_t(depth | kEscaped);
         } else {
             const uint32_t m = (h.info >> 16) & 0xffu;
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
                 vmat[depth * P + p] = uint8_t(m);
                 vstat[p] = uint8_t(depth + 1);
             }
-            for (int a = 0; a < 3; a++) L[a * P + p] += b[a] * d.emission[m * 3 + a];
+            for (int a = 0; a < 3; a++) l[a] += b[a] * d.emission[m * 3 + a];
             if (depth < d.bounces) {
                 const cqx_ray r = rays_cur[i];
                 float pos[3], nrm[3];
@@ -247,6 +251,7 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
                 }
             }
         }
+        for (int a = 0; a < 3; a++) L[a * P + p] = l[a];
     }
     // compact the surviving rays
     const bool take[1] = { cont };
@@ -573,8 +578,7 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const uint32_t gb = uint32_t((P + 255) / 256);
     const uint32_t gc = uint32_t((P + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
-    hipLaunchKernelGGL(k_pt_generate, dim3(gb), dim3(256), 0, s, p->dparams, P, p->rays[0], p->paths[0], p->beta, p->L,
-                       p->rng, p->counts + 0);
+    hipLaunchKernelGGL(k_pt_generate, dim3(gb), dim3(256), 0, s, p->dparams, P, p->rays[0], p->rng, p->counts + 0);
     cqx_status cst = CQX_OK;
     for (uint32_t depth = 0; depth <= p->bounces && cst == CQX_OK; depth++) {
         const int cur = depth & 1, nxt = cur ^ 1;
