@@ -415,7 +415,8 @@ struct PtPlan {
     const cqx_dvolume* vol = nullptr;
     uint32_t width = 0, height = 0, spp = 0, bounces = 0;
     float max_footprint = -1.0f;
-    cqx_launch_cfg cfg{};
+    cqx_launch_cfg cfg{};  // traversal launch of the bounce rays
+    cqx_launch_cfg cfg0{}; // traversal launch of the camera rays (depth 0)
     uint64_t P = 0;
     PtDevParams* dparams = nullptr;
     cqx_ray* rays[2] = { nullptr, nullptr };
@@ -520,9 +521,19 @@ cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* p
     cqx_launch_cfg c{};
     if (cfg) c = *cfg;
     c.bin_bits = 0; // the bounce launches read their ray count on the device: no binning prepass
-    // "auto" (variant 0): persistent waves at 24 per CU, the best of a sweep on the 1024^3,
-    // 1280x720x16 frame (4.13 ms against 4.38 at 32 per CU, 4.41 at 16)
-    if (c.variant == 0 && !c.waves_per_cu) c.waves_per_cu = 24;
+    // "auto" (variant 0): the camera rays (depth 0, coherent: a pixel's samples are adjacent) one
+    // per lane, 64 per workgroup; the bounce rays on persistent waves at 24 per CU. On the 1024^3
+    // 1280x720x16 frame's camera rays one ray per lane took 0.77 ms against 0.90 ms persistent;
+    // over the whole frame persistent waves at 24 per CU gave 4.13 ms against 4.38 at 32 per CU,
+    // 4.41 at 16 and 5.1 with one ray per lane.
+    cqx_launch_cfg c0 = c;
+    if (c.variant == 0) {
+        c0 = cqx_launch_cfg{};
+        c0.variant = 1;
+        c0.block = 64;
+        c0.max_iters = c.max_iters;
+        if (!c.waves_per_cu) c.waves_per_cu = 24;
+    }
     PtPlan* p = ctx->pt_plan;
     if (p && p->vol == vol && p->width == pp->width && p->height == pp->height && p->spp == pp->spp &&
         p->bounces == pp->bounces && p->max_footprint == pp->max_footprint &&
@@ -542,6 +553,7 @@ cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* p
     p->bounces = pp->bounces;
     p->max_footprint = pp->max_footprint;
     p->cfg = c;
+    p->cfg0 = c0;
     p->P = uint64_t(pp->width) * pp->height * pp->spp;
     cqx_status st = alloc(&p->dparams, 1);
     if (st == CQX_OK) st = alloc(&p->L, 3 * p->P);
@@ -586,7 +598,7 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
         uint32_t* cnt_next = p->counts + nxt * 16;
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, cnt_next, 0u);
         cst = cqx::launch_intersect(ctx, p->vol, p->rays[cur], P, cnt_cur, CQX_SURFACE_PROPERTIES, p->max_footprint,
-                                    p->hits, &p->cfg, s);
+                                    p->hits, depth == 0 ? &p->cfg0 : &p->cfg, s);
         hipLaunchKernelGGL(k_pt_shade, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
                            p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next,
                            p->vmat, p->vstat);

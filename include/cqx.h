@@ -384,8 +384,9 @@ typedef struct {
 
 /* mode 0: wavefront (traversal kernel per bounce, captured in a hipGraph, replayed per call);
  * mode 1: megakernel (one lane per path). Both give the same image bit for bit. `image` is a
- * device pointer; synchronizes `stream` before returning. cfg NULL or variant 0: the bounce
- * launches use persistent waves, 24 per CU unless cfg sets waves_per_cu. */
+ * device pointer; synchronizes `stream` before returning. cfg NULL or variant 0: the camera
+ * rays are traced one per lane (64 per workgroup), the bounce rays on persistent waves (24 per
+ * CU unless cfg sets waves_per_cu). An explicit variant applies to every depth. */
 cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params, uint32_t mode,
                          const cqx_launch_cfg* cfg, float* image, void* stream);
 
