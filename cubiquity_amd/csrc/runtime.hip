@@ -38,11 +38,17 @@ __global__ void __launch_bounds__(256) k_intersect_lane(TraceParams P, const cqx
     }
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // rays are read once and records written once: streaming accesses keep node lines in L2
     const float* r = reinterpret_cast<const float*>(rays + i);
-    const float ox = r[0], oy = r[1], oz = r[2], dx = r[3], dy = r[4], dz = r[5];
+    float v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = __builtin_nontemporal_load(r + k);
     cqx_hit h;
-    trace_ray(P, ox, oy, oz, dx, dy, dz, lds_stack + threadIdx.x, blockDim.x, h);
-    hits[i] = h;
+    trace_ray(P, v[0], v[1], v[2], v[3], v[4], v[5], lds_stack + threadIdx.x, blockDim.x, h);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&h);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(hits + i);
+#pragma unroll
+    for (int k = 0; k < 8; k++) __builtin_nontemporal_store(src[k], dst + k);
 }
 
 // ---- variant 2: persistent waves with ballot refill ------------------------------------------
