@@ -217,3 +217,15 @@ def test_gpu_resolve_both_paths(cq, oracle, gpu_ctx, scene, dscene, spp):
     img = cq.pathtrace(gpu_ctx, dscene, s, 0).cpu().numpy()
     ref = oracle.pathtrace(hv.nodes, hv.root, s.params)
     assert np.array_equal(img, ref), f"spp {spp}: {(img != ref).sum()} values differ"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_odd_image_size(cq, oracle, gpu_ctx, scene, dscene, mode):
+    """37 x 23 pixels x 3 spp = 2553 paths: the shading pass's 1024-thread compaction blocks end
+    part-full at every depth, and the resolve's last 256-pixel block holds 83 pixels."""
+    hv, s0 = scene
+    s = cq.PathTraceSetup(W.RAYCAST_128.camera, 37, 23, 3, 4, s0.albedo, s0.emission)
+    img = cq.pathtrace(gpu_ctx, dscene, s, mode).cpu().numpy()
+    ref = oracle.pathtrace(hv.nodes, hv.root, s.params)
+    assert np.array_equal(img, ref), f"mode {mode}: {(img != ref).sum()} values differ"
