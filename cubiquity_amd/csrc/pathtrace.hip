@@ -20,9 +20,9 @@
 //              1024-thread block
 //   megakernel one lane per path, grid-stride
 // Both write per-path radiance; k_pt_resolve sums the samples of a pixel in sample order.
-// Adjoint (path replay): replays the wavefront pass, which records one byte per path vertex (its
-// material) and how the path ended (the PCG streams make the replay the same paths), then a sweep
-// kernel rebuilds each path's throughputs in the forward's FP32 order and runs backwards,
+// Adjoint (path replay): runs the wavefront pass again (each path's PCG stream makes it trace the
+// same paths), which records one byte per path vertex (its material) and how the path ended; then
+// a sweep kernel rebuilds each path's throughputs in the forward's FP32 order and runs backwards,
 //   G_K = sky if escaped else 0;  G_j = E_j + A_j G_{j+1};
 //   dV/dE_j = beta_j,  dV/dA_j = beta_j G_{j+1},
 // scaled by dL/dpixel / spp; per-block partial sums in LDS, reduced over blocks in FP64.
