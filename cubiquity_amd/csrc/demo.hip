@@ -493,19 +493,24 @@ cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* r
     d.seed = params->seed;
     const float fp = params->max_footprint;
 
-    // "auto" (cfg null or variant 0): one ray per lane, 128 per workgroup. The frame's batches
-    // (a ray per pixel, then the shadow and reflected rays of the hits) are too small for
-    // persistent waves to pay: on the 1280x720 benchmark frame this takes 0.90 ms against 1.04 ms
-    // with the persistent default (32 waves per CU).
+    // "auto" (cfg null or variant 0): the primaries (coherent camera rays) one per lane, 64 per
+    // workgroup; the secondary lists (incoherent) on a light persistent grid, 16 waves per CU.
+    // The frame's batches are too small for the persistent default (32 waves per CU) to pay.
     cqx_launch_cfg c1{};
     if (cfg) c1 = *cfg;
+    cqx_launch_cfg c2 = c1;
     if (c1.variant == 0) {
+        c1 = cqx_launch_cfg{};
         c1.variant = 1;
-        if (!c1.block) c1.block = 128;
-        c1.bin_bits = 0;
+        c1.block = 64;
+        c1.max_iters = c2.max_iters;
+        const uint32_t max_iters = c2.max_iters;
+        c2 = cqx_launch_cfg{};
+        c2.variant = 2;
+        c2.waves_per_cu = 16;
+        c2.max_iters = max_iters;
     }
     // secondary launches read their ray count on the device: no binning prepass there
-    cqx_launch_cfg c2 = c1;
     c2.bin_bits = 0;
     const uint32_t g1 = uint32_t((n + 255) / 256), g2 = uint32_t((2 * n + 255) / 256);
     const uint32_t gc = uint32_t((n + cqx::kCompactBlock - 1) / cqx::kCompactBlock);
