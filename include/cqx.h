@@ -416,7 +416,8 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
  * anomalies count as misses.
  * rays: device, n x 6 floats (the camera rays of PathtracingDemo::raytrace, :214-225); out_rgb:
  * device, n x 3 floats, overwritten or, with accumulate, added to (the demo's mImage +=).
- * cfg NULL or variant 0: one ray per lane, 128 per workgroup, for every traversal of the frame.
+ * cfg NULL or variant 0: the primaries one per lane (64 per workgroup), the secondary rays on
+ * persistent waves (16 per CU). An explicit variant applies to every traversal of the frame.
  * Asynchronous on `stream`; one call in flight per context. n <= 2^30. */
 typedef struct {
     const float* colours; /* host, 256 x 3: the material colours (cqx_materials_colour_table) */
