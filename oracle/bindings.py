@@ -48,10 +48,14 @@ class _DemoParams(ctypes.Structure):
 class Oracle:
     """The C restatement (cpu_ref.c / cpu_gen.c)."""
 
-    def __init__(self):
-        if not os.path.exists(ORACLE_LIB):
-            build(ref=False)
-        L = ctypes.CDLL(ORACLE_LIB)
+    def __init__(self, lib_path=None):
+        """lib_path: another build of the same sources (the mutation tests load planted-slip
+        builds this way); default liboracle.so."""
+        if lib_path is None:
+            lib_path = ORACLE_LIB
+            if not os.path.exists(lib_path):
+                build(ref=False)
+        L = ctypes.CDLL(lib_path)
         sig = {
             "orc_find_subdags": (ctypes.c_int, [_P, _U64, _U32, _P]),
             "orc_intersect": (None, [_P, _P, _P, _U64, ctypes.c_int, ctypes.c_float, _U32, _P]),
@@ -71,6 +75,11 @@ class Oracle:
             "orc_compare_dense": (_U64, [_P, ctypes.c_int, _P, _U32, ctypes.c_int]),
             "orc_pathtrace": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int]),
             "orc_pathtrace_f64": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]),
+            "orc_pcg32": (None, [_U64, _U64, _U32, _P]),
+            "orc_pcg_uniform": (None, [_U64, _U64, _U32, _P]),
+            "orc_sincos2pi": (None, [_P, _U64, _P, _P]),
+            "orc_pt_bounce_rays": (_U32, [_P, _P, ctypes.c_float, _U64, _U64, _U32, _P]),
+            "orc_pt_camera_ray": (ctypes.c_int, [_P, _U64, _P, _P]),
             "orc_bit_mix": (_U64, [_U64]),
             "orc_fnv1a": (_U64, [_P, _I64, _U64]),
             "orc_position_noise": (ctypes.c_float, [_P]),
@@ -125,6 +134,41 @@ class Oracle:
     def voxel(self, nodes, root, x, y, z):
         nodes = self._nodes(nodes)
         return int(self.L.orc_voxel(_ptr(nodes), root, int(x), int(y), int(z)))
+
+    # ---- pieces of the path tracer (cpu_pt.c), for the pinning tests ----
+    def pcg32(self, initstate, initseq, n):
+        out = np.zeros(n, np.uint32)
+        self.L.orc_pcg32(initstate, initseq, n, _ptr(out))
+        return out
+
+    def pcg_uniform(self, initstate, initseq, n):
+        out = np.zeros(n, np.float32)
+        self.L.orc_pcg_uniform(initstate, initseq, n, _ptr(out))
+        return out
+
+    def sincos2pi(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        c = np.zeros_like(u)
+        s = np.zeros_like(u)
+        self.L.orc_sincos2pi(_ptr(u), u.size, _ptr(c), _ptr(s))
+        return c, s
+
+    def bounce_rays(self, normal, pos, offset, initstate, initseq, count):
+        """(count, 6) rays: cosine-weighted bounces about `normal` from `pos`, one PCG stream."""
+        n = np.ascontiguousarray(normal, dtype=np.float32).reshape(3)
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(3)
+        out = np.zeros((count, 6), np.float32)
+        k = self.L.orc_pt_bounce_rays(_ptr(n), _ptr(p), ctypes.c_float(offset), initstate, initseq, count,
+                                      _ptr(out))
+        return out[:k]
+
+    def pt_camera_ray(self, params, path):
+        """(ray, jitter) of path number `path`."""
+        ray = np.zeros(6, np.float32)
+        jit = np.zeros(2, np.float32)
+        if self.L.orc_pt_camera_ray(ctypes.byref(params), path, _ptr(ray), _ptr(jit)) != 0:
+            raise ValueError("bad path-tracer parameters")
+        return ray, jit
 
     # ---- the reference CPU tracer's shading (cpu_demo.c) ----
     def bit_mix(self, x):

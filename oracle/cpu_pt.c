@@ -11,9 +11,17 @@
  * the material tables) but accumulates throughput, radiance and gradients in FP64; it is the
  * reference for the GPU adjoint, and the image it returns bounds the GPU's FP32 rounding.
  *
- * Pinning: the gradient is checked against central finite differences of the FP64 image (a
- * polynomial in the tables), the image against closed forms (an empty volume shows the sky
- * everywhere; a material's emission enters linearly).
+ * Pinning (each piece by something outside this file):
+ *   PCG32        O'Neill's published pcg32 demo stream (initstate 42, initseq 54);
+ *   sincos2pi    every FP32 input k / 2^24 of the uniform draws against FP64 sin/cos;
+ *   bounce       moments of the cosine-weighted law (unit length, n.d >= 0, E[n.d] = 2/3,
+ *                E[(n.d)^2] = 1/2, the CDF of n.d, uniform azimuth) on axis and edge normals;
+ *   camera ray   the FP64 pinhole through the jittered pixel point (camera.cpp:12-38 geometry);
+ *   estimator    closed forms: a furnace (camera inside a closed shell of one material gives
+ *                e (1 + a + ... + a^B) per pixel), a lit floor (e + a sky for B >= 1, e for
+ *                B = 0), an empty volume (sky), emission linearity;
+ *   gradient     central finite differences of the FP64 image (a polynomial in the tables).
+ * A planted slip in any of these (tests/test_oracle_mutations.py) fails one of the checks.
  */
 #include "oracle.h"
 
@@ -317,5 +325,54 @@ int orc_pathtrace_f64(const uint32_t* nodes, const orc_subdag* subdags, const or
             if (d_albedo) d_albedo[(m - 1) * 3 + a] = grad[m * 6 + a];
             if (d_emission) d_emission[(m - 1) * 3 + a] = grad[m * 6 + 3 + a];
         }
+    return 0;
+}
+
+/* ---- entry points for the pinning tests (each runs one piece of the path tracer alone) ---- */
+
+void orc_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out)
+{
+    pcg_t r = pcg_init(initstate, initseq);
+    for (uint32_t i = 0; i < n; i++) out[i] = pcg_step(&r);
+}
+
+void orc_pcg_uniform(uint64_t initstate, uint64_t initseq, uint32_t n, float* out)
+{
+    pcg_t r = pcg_init(initstate, initseq);
+    for (uint32_t i = 0; i < n; i++) out[i] = pcg_u(&r);
+}
+
+void orc_sincos2pi(const float* u, uint64_t n, float* c, float* s)
+{
+    for (uint64_t i = 0; i < n; i++) sincos2pi(u[i], &c[i], &s[i]);
+}
+
+/* `count` bounces about the face normal n from the origin `pos`, all drawing from one PCG stream
+ * (two draws each, as along a path); rays: count x 6. Returns the number of rays written (0 for a
+ * zero normal). */
+uint32_t orc_pt_bounce_rays(const float n[3], const float pos[3], float bounce_offset, uint64_t initstate,
+                            uint64_t initseq, uint32_t count, float* rays)
+{
+    orc_pt_params p;
+    memset(&p, 0, sizeof(p));
+    p.bounce_offset = bounce_offset;
+    pcg_t r = pcg_init(initstate, initseq);
+    for (uint32_t i = 0; i < count; i++)
+        if (!bounce(&p, &r, pos, n, rays + 6 * (uint64_t)i)) return i;
+    return count;
+}
+
+/* The camera ray of path number `path` and the pixel jitter (jx, jy) it was drawn with. */
+int orc_pt_camera_ray(const orc_pt_params* p, uint64_t path, float ray[6], float jitter[2])
+{
+    cam_t* c = (cam_t*)malloc(sizeof(cam_t));
+    if (!c || setup(p, c) != 0) { free(c); return -1; }
+    pcg_t rng;
+    uint32_t pixel;
+    start(p, c, path, &rng, ray, &pixel);
+    pcg_t again = pcg_init((p->seed << 32) + (uint64_t)pixel, (uint64_t)(path % p->spp));
+    jitter[0] = pcg_u(&again);
+    jitter[1] = pcg_u(&again);
+    free(c);
     return 0;
 }

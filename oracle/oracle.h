@@ -119,6 +119,14 @@ int orc_pathtrace(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt
 int orc_pathtrace_f64(const uint32_t* nodes, const orc_subdag* subdags, const orc_pt_params* p,
                       const double* d_image, double* image, double* d_albedo, double* d_emission, int threads);
 
+/* pieces of the path tracer, for the pinning tests */
+void orc_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out);
+void orc_pcg_uniform(uint64_t initstate, uint64_t initseq, uint32_t n, float* out);
+void orc_sincos2pi(const float* u, uint64_t n, float* c, float* s);
+uint32_t orc_pt_bounce_rays(const float n[3], const float pos[3], float bounce_offset, uint64_t initstate,
+                            uint64_t initseq, uint32_t count, float* rays);
+int orc_pt_camera_ray(const orc_pt_params* p, uint64_t path, float ray[6], float jitter[2]);
+
 /* ---- the reference CPU tracer's shading (cpu_demo.c) ---- */
 /* same field order as the product's cqx_demo_params */
 typedef struct {

@@ -12,7 +12,15 @@ Contents (all produced by the reference's own intersectVolume / traceRayRef / fi
   camera_rays          all 65,536 camera rays
   camera_ref_sha256    SHA-256 of the reference's records for all camera rays (surface on)
 
-Run: python tests/golden/make_golden.py   (needs the reference checkout to build oracle/_ref)
+lattice_128_golden.npz (python tests/golden/make_golden.py --lattice), same volume:
+  rays                 4096 lattice rays (tests/raysets.py lattice_rays, seed 77) on which the
+                       reference terminates (rays with a zero component whose origin lies on a
+                       node boundary can loop forever in intersectRayNodeESVO; the oracle's
+                       iteration cap finds them beforehand and they are left out)
+  ref_surf1, ref_surf0 intersectVolume results (surface properties on / off)
+  ref_lod_*            intersectVolume results at maxFootprint 0.05, 0.3 and 2.0
+
+Run: python tests/golden/make_golden.py [--lattice]   (needs the reference checkout to build oracle/_ref)
 """
 import hashlib
 import os
@@ -27,9 +35,38 @@ sys.path.insert(0, ROOT)
 import cubiquity_amd as cq  # noqa: E402
 from cubiquity_amd import workloads as W  # noqa: E402
 from oracle.bindings import Reference, build  # noqa: E402
-from tests.raysets import special_rays  # noqa: E402
+from tests.raysets import lattice_rays, special_rays  # noqa: E402
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "raycast_128_golden.npz")
+OUT_LATTICE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lattice_128_golden.npz")
+LATTICE_LODS = (0.05, 0.3, 2.0)
+
+
+def lattice():
+    from oracle.bindings import Oracle
+
+    build(ref=True)
+    ref = Reference()
+    orc = Oracle()
+    w = W.RAYCAST_128
+    hv = cq.HostVolume.generate(w.params())
+    rays = lattice_rays(w.side, 4096 + 512, 77)
+    sub = orc.find_subdags(hv.nodes, hv.root)
+    ok = np.ones(rays.shape[0], bool)
+    for surf, fp in [(True, -1.0)] + [(True, f) for f in LATTICE_LODS]:
+        h = orc.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+        ok &= ((h["info"] >> 15) & 1) == 0
+    rays = rays[ok][:4096]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "v.dag")
+        hv.save_dag(path)
+        rv = ref.load(path)
+        out = {"dag_sha256": hashlib.sha256(hv.nodes.tobytes()).hexdigest(), "rays": rays,
+               "ref_surf1": rv.intersect(rays, True), "ref_surf0": rv.intersect(rays, False)}
+        for fp in LATTICE_LODS:
+            out[f"ref_lod_{fp}"] = rv.intersect(rays, True, fp)
+        np.savez_compressed(OUT_LATTICE, **out)
+    print(OUT_LATTICE, os.path.getsize(OUT_LATTICE), "bytes")
 
 
 def main():
@@ -70,4 +107,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    lattice() if "--lattice" in sys.argv else main()

@@ -101,3 +101,23 @@ def deep_volumes(cq):
         rays.append(np.concatenate([rng.uniform(-8, 8, (400, 3)), d / np.linalg.norm(d, axis=1, keepdims=True)], 1))
         out[name] = (hv, np.concatenate(rays).astype(np.float32))
     return out
+
+
+def lattice_rays(n, count, seed):
+    """Rays whose traversal arithmetic is exact in FP32: origins on the voxel-corner lattice
+    (integer - 0.5, so the reference's shifted origin o + 0.5 is an integer) and direction
+    components in {0, +-1/4, +-1/2, +-1, +-2}, signed towards the volume's centre (about one ray in
+    ten has a zero component, with +0 or -0). Child boundaries, entry and exit times are then
+    dyadic, so entries on a child's centre plane, edge and corner crossings and equal axis
+    distances (the `<` / `<=` ties of findFirstChild, the sibling advance and the octant walk) occur
+    on most rays, and zero components on a node boundary give the NaN operands whose order the
+    min/max helpers keep. Not normalised (intersectVolume does not need unit directions)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    o = rng.integers(-n // 2, n + n // 2, (count, 3)).astype(np.float64) - 0.5
+    d = rng.choice(np.array([0.25, 0.5, 1.0, 2.0]), (count, 3))
+    d = d * np.sign(n / 2.0 - o + 0.01)
+    zero = rng.random(count) < 0.1
+    axis = rng.integers(0, 3, count)
+    neg = rng.random(count) < 0.5
+    d[zero, axis[zero]] = np.where(neg[zero], -0.0, 0.0)
+    return np.concatenate([o, d], 1).astype(np.float32)

@@ -106,6 +106,27 @@ def test_gpu_matches_golden(cq, gpu_ctx, dvol128, vol128):
     assert hashlib.sha256(res.tobytes()).hexdigest() == str(z["camera_ref_sha256"])
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+def test_gpu_matches_lattice_golden(cq, oracle, gpu_ctx, dvol128, vol128, variant):
+    """Lattice rays (exact FP32 ties at child boundaries, +-0 components on node boundaries)
+    against the compiled reference's records, surface properties on and off and at three LOD
+    footprints, and the full 32-byte records against the oracle."""
+    from tests import pt_pins
+
+    lat = np.load(pt_pins.LATTICE, allow_pickle=False)
+    assert hashlib.sha256(vol128.nodes.tobytes()).hexdigest() == str(lat["dag_sha256"])
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    for name, rays, surf, fp, ref in pt_pins.golden_cases(np.load(GOLDEN, allow_pickle=False), lat):
+        if not name.startswith("lattice/"):
+            continue
+        g = _gpu(gpu_ctx, dvol128, rays, surf, fp, cq.LaunchCfg(variant=variant))
+        eq = records_equal(cq.hits_to_intersections(rays, g, surf), ref)
+        assert eq.all(), f"{name}: {(~eq).sum()} rays differ from the reference fixture"
+        o = oracle.intersect(vol128.nodes, sub, rays, surf, fp, threads=8)
+        eq = records_equal(g, o)
+        assert eq.all(), _report(name, eq, rays, g, o)
+
+
 def test_gpu_matches_reference_live(cq, reference, gpu_ctx, dvol128, dag128, rays128):
     rv = reference.load(dag128)
     for name in ("camera", "incoherent", "special"):

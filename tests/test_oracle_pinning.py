@@ -7,7 +7,10 @@
   * against closed forms: axis-aligned and diagonal rays into a single solid box;
   * the derived outputs (hit voxel, parent node, slot) against the closed-form root descent of
     Volume::voxel (src/library/storage.cpp:481-499).
-A planted mistake (tests/test_oracle_mutations.py) shows these checks catch plausible slips.
+  * exact-tie lattice rays (tests/raysets.py lattice_rays) against the compiled reference's
+    records in tests/golden/lattice_128_golden.npz.
+Planted slips (tests/test_oracle_mutations.py: a flipped comparison, operand order, sign, shift
+or table entry, each in a copy of the oracle) each fail one of these checks.
 """
 import hashlib
 import os
@@ -81,6 +84,28 @@ def test_oracle_vs_golden(oracle, vol128, golden):
     g = golden["subdags"]
     for f in ("lower", "height", "node"):
         assert (sub[f] == g[f]).all()
+
+
+def test_oracle_vs_lattice_golden(oracle, vol128, golden):
+    from tests import pt_pins
+
+    pt_pins.check_traversal_golden(oracle, vol128, golden, np.load(pt_pins.LATTICE, allow_pickle=False))
+
+
+def test_oracle_vs_reference_lattice_live(oracle, reference, vol128, dag128):
+    """A fresh lattice set (another seed) against the live reference; rays the oracle's cap flags
+    (the reference's non-terminating cases) are left out before the reference sees them."""
+    from tests.raysets import lattice_rays
+
+    rays = lattice_rays(128, 3000, 78)
+    sub = _sub(oracle, vol128)
+    h = oracle.intersect(vol128.nodes, sub, rays, True, -1.0, max_iters=1 << 16)
+    rays = rays[((h["info"] >> 15) & 1) == 0]
+    rv = reference.load(dag128)
+    for surf, fp in ((True, -1.0), (False, -1.0), (True, 0.3)):
+        o = oracle.to_results(rays, oracle.intersect(vol128.nodes, sub, rays, surf, fp), surf)
+        eq = records_equal(o, rv.intersect(rays, surf, fp))
+        assert eq.all(), (surf, fp, (~eq).sum())
 
 
 # ---- against an independent FP64 dense brute force ---------------------------------------------
