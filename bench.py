@@ -282,9 +282,23 @@ def bench_large(cq, W, ctx, frames, w=None):
     hv = cq.HostVolume.generate(w.params())
     build_s = time.time() - t0
     log(f"large: {hv.node_count} nodes ({hv.node_count * 32 / 1e9:.2f} GB) built in {build_s:.1f}s")
-    rays = torch.from_numpy(cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)).cuda()
+    rays_host = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)
+    rays = torch.from_numpy(rays_host).cuda()
     hits = torch.empty((rays.shape[0], 32), dtype=torch.uint8, device="cuda")
     log("large: rays ready")
+
+    def oracle_sample(n=1 << 20):
+        """Records of the first n rays of the last launch against the oracle on the current store
+        (bit for bit); returns the number of equal records."""
+        from oracle.bindings import Oracle
+
+        orc = Oracle()
+        sub = orc.find_subdags(hv.nodes, hv.root)
+        ref = orc.intersect(hv.nodes, sub, rays_host[:n], True, -1.0, threads=cpu_threads())
+        got = hits[:n].cpu().numpy()
+        eq = (got.reshape(n, 32) == ref.view(np.uint8).reshape(n, 32)).all(axis=1)
+        log(f"large: oracle sample of {n} rays: {int(eq.sum())} equal")
+        return int(eq.sum())
     t0 = time.time()
     dv = ctx.upload(hv)
     upload_s = time.time() - t0
@@ -297,6 +311,7 @@ def bench_large(cq, W, ctx, frames, w=None):
                        float((hits.view(torch.int32)[:, 5] & 1).to(torch.float64).mean().item()))
     entries0, hitfrac0 = per_ray()
     log(f"large: traversal {before:.2f} ms, {entries0:.2f} node entries per ray, hit fraction {hitfrac0:.4f}")
+    sample_before = oracle_sample()
     nodes0 = hv.node_count
     edit_ms, update_ms, trace_ms, appended = [], [], [], []
     for f in range(frames):
@@ -312,6 +327,7 @@ def bench_large(cq, W, ctx, frames, w=None):
         entries1, hitfrac1 = per_ray()
         log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms, update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms, "
             f"{entries1:.2f} node entries per ray, hit fraction {hitfrac1:.4f}")
+    sample_after = oracle_sample()
     dv.free()
     mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
     n = rays.shape[0]
@@ -324,6 +340,8 @@ def bench_large(cq, W, ctx, frames, w=None):
             "trace_ms_after_edits": mean(trace_ms),
             "node_entries_per_ray": round(entries0, 3), "node_entries_per_ray_after_edits": round(entries1, 3),
             "hit_fraction": round(hitfrac0, 5), "hit_fraction_after_edits": round(hitfrac1, 5),
+            "oracle_sample_rays": 1 << 20, "oracle_sample_equal_before_edits": sample_before,
+            "oracle_sample_equal_after_edits": sample_after,
             "frame_ms_steady": round(float(np.mean(edit_ms[1:] or edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 1),
             "launch": {f2: getattr(best, f2) for f2, _ in best._fields_}}
 
