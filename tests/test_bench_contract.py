@@ -29,13 +29,26 @@ def test_edit_frames_are_seeded_by_frame():
 
 
 def test_traffic_profile_is_consistent():
+    """Every profiled launch's calibrated HBM bytes follow from its counters; the bench fills
+    traffic and the L2 hit rate only for the launch it actually timed."""
+    sys.path.insert(0, ROOT)
+    import bench
+
     d = json.load(open(os.path.join(ROOT, "profiles", "traffic_intersect_incoherent_1024.json")))
-    cal = d["calibration"]
-    assert abs(cal["fetch_factor"] - cal["copy_bytes"] / 1024 / cal["copy_fetch_size_kib"]) < 1e-4
-    assert abs(cal["write_factor"] - cal["copy_bytes"] / 1024 / cal["copy_write_size_kib"]) < 1e-4
-    hbm = d["fetch_size_kib_per_launch"] * 1024 * cal["fetch_factor"] + \
-        d["write_size_kib_per_launch"] * 1024 * cal["write_factor"]
-    assert abs(hbm - d["hbm_bytes_per_launch"]) / hbm < 1e-3
-    assert 0 < d["l2_hit_rate"] < 1
-    # the records alone are 2^24 x 32 bytes
-    assert d["write_size_kib_per_launch"] * 1024 * cal["write_factor"] >= (1 << 24) * 32
+    assert d["launches"]
+    for p in d["launches"]:
+        cal = p["calibration"]
+        assert abs(cal["fetch_factor"] - cal["copy_bytes"] / 1024 / cal["copy_fetch_size_kib"]) < 1e-4
+        assert abs(cal["write_factor"] - cal["copy_bytes"] / 1024 / cal["copy_write_size_kib"]) < 1e-4
+        hbm = p["fetch_size_kib_per_launch"] * 1024 * cal["fetch_factor"] + \
+            p["write_size_kib_per_launch"] * 1024 * cal["write_factor"]
+        assert abs(hbm - p["hbm_bytes_per_launch"]) / hbm < 1e-3
+        assert 0 < p["l2_hit_rate"] < 1
+        # the records alone are 2^24 x 32 bytes
+        assert p["write_size_kib_per_launch"] * 1024 * cal["write_factor"] >= (1 << 24) * 32
+        cfg = dict(kv.split("=") for kv in p["launch"].split(","))
+        assert p["launch"] == bench.launch_string({k: int(v) for k, v in cfg.items()})
+        traffic, l2, prof = bench.traffic_from_profiles(p["launch"])
+        assert traffic == p["hbm_bytes_per_launch"] and l2 == p["l2_hit_rate"] and prof
+    assert bench.traffic_from_profiles("variant=3,block=128,waves_per_cu=7,regroup_interval=2,refill_lanes=3,"
+                                       "lds_nodes=0,bin_bits=0") == (None, None, None)
