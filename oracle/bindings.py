@@ -31,7 +31,7 @@ _I64 = ctypes.c_int64
 
 
 def build(ref=True):
-    targets = ["oracle"] + (["ref"] if ref and os.path.isdir(REF_SRC) else [])
+    targets = ["oracle"] + (["ref", "dropin"] if ref and os.path.isdir(REF_SRC) else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 
