@@ -5,7 +5,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcqx.so")
+# CQX_LIBRARY names another build of the same sources (the sanitizer build of the host code,
+# cubiquity_amd/build.py --sanitize); by default the in-tree libcqx.so.
+LIB_PATH = os.environ.get("CQX_LIBRARY") or os.path.join(HERE, "libcqx.so")
 
 CQX_SURFACE_PROPERTIES = 0x1
 MAX_FOOTPRINT_DISABLED = -1.0

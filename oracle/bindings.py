@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+# CQX_ORACLE_LIBRARY names another build of the same sources (make -C oracle asan)
+ORACLE_LIB = os.environ.get("CQX_ORACLE_LIBRARY") or os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libcubiquity_ref.so")
 REF_SRC = "/root/reference/src/library"
 
