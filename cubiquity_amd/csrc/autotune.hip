@@ -44,9 +44,6 @@ const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n)
     for (const auto& t : ctx->tuned) {
         const int dh = std::abs(t.height - height), dc = std::abs(t.cls - cls);
         if (dh > 2 || dc > 2) continue;
-        // variant 4's LDS node copy was sized beside the tuned volume's stack; a deeper volume's
-        // stack may leave too little LDS for it
-        if (dh > 0 && t.cfg.variant == 4) continue;
         const int d = dh * 8 + dc;
         if (d < best_d) {
             best_d = d;

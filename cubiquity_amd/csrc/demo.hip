@@ -496,19 +496,15 @@ cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* r
     // "auto" (cfg null or variant 0): the primaries (coherent camera rays) one per lane, 64 per
     // workgroup; the secondary lists (incoherent) on a light persistent grid, 16 waves per CU.
     // The frame's batches are too small for the persistent default (32 waves per CU) to pay.
+    // Only the fields the caller left at zero take these values.
     cqx_launch_cfg c1{};
     if (cfg) c1 = *cfg;
     cqx_launch_cfg c2 = c1;
     if (c1.variant == 0) {
-        c1 = cqx_launch_cfg{};
         c1.variant = 1;
-        c1.block = 64;
-        c1.max_iters = c2.max_iters;
-        const uint32_t max_iters = c2.max_iters;
-        c2 = cqx_launch_cfg{};
+        if (!c1.block) c1.block = 64;
         c2.variant = 2;
-        c2.waves_per_cu = 16;
-        c2.max_iters = max_iters;
+        if (!c2.waves_per_cu) c2.waves_per_cu = 16;
     }
     // secondary launches read their ray count on the device: no binning prepass there
     c2.bin_bits = 0;

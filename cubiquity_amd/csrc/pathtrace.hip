@@ -526,12 +526,11 @@ cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* p
     // 1280x720x16 frame's camera rays one ray per lane took 0.77 ms against 0.90 ms persistent;
     // over the whole frame persistent waves at 24 per CU gave 4.13 ms against 4.38 at 32 per CU,
     // 4.41 at 16 and 5.1 with one ray per lane.
+    // Only the fields the caller left at zero take these values.
     cqx_launch_cfg c0 = c;
     if (c.variant == 0) {
-        c0 = cqx_launch_cfg{};
         c0.variant = 1;
-        c0.block = 64;
-        c0.max_iters = c.max_iters;
+        if (!c0.block) c0.block = 64;
         if (!c.waves_per_cu) c.waves_per_cu = 24;
     }
     PtPlan* p = ctx->pt_plan;

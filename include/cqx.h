@@ -385,8 +385,9 @@ typedef struct {
 /* mode 0: wavefront (traversal kernel per bounce, captured in a hipGraph, replayed per call);
  * mode 1: megakernel (one lane per path). Both give the same image bit for bit. `image` is a
  * device pointer; synchronizes `stream` before returning. cfg NULL or variant 0: the camera
- * rays are traced one per lane (64 per workgroup), the bounce rays on persistent waves (24 per
- * CU unless cfg sets waves_per_cu). An explicit variant applies to every depth. */
+ * rays are traced one per lane (64 per workgroup unless cfg sets block), the bounce rays on
+ * persistent waves (24 per CU unless cfg sets waves_per_cu); the other fields of cfg apply as
+ * given. An explicit variant applies to every depth. */
 cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params, uint32_t mode,
                          const cqx_launch_cfg* cfg, float* image, void* stream);
 
@@ -416,8 +417,9 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
  * anomalies count as misses.
  * rays: device, n x 6 floats (the camera rays of PathtracingDemo::raytrace, :214-225); out_rgb:
  * device, n x 3 floats, overwritten or, with accumulate, added to (the demo's mImage +=).
- * cfg NULL or variant 0: the primaries one per lane (64 per workgroup), the secondary rays on
- * persistent waves (16 per CU). An explicit variant applies to every traversal of the frame.
+ * cfg NULL or variant 0: the primaries one per lane (64 per workgroup unless cfg sets block),
+ * the secondary rays on persistent waves (16 per CU unless cfg sets waves_per_cu); the other
+ * fields of cfg apply as given. An explicit variant applies to every traversal of the frame.
  * Asynchronous on `stream`; one call in flight per context. n <= 2^30. */
 typedef struct {
     const float* colours; /* host, 256 x 3: the material colours (cqx_materials_colour_table) */
