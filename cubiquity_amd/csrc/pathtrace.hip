@@ -332,30 +332,46 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 
 // The backward sweep over each path's recorded vertices. The throughputs beta_j are rebuilt as the
 // forward pass formed them (beta_0 = 1, beta_{j+1} = beta_j * albedo[m_j], FP32 in that order).
-__global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restrict__ dp, uint64_t P,
-                                                    const uint8_t* __restrict__ vmat,
-                                                    const uint8_t* __restrict__ vstat,
-                                                    const float* __restrict__ dimage, uint32_t M,
-                                                    float* __restrict__ partials)
+//
+// The gradient sums have a fixed order, so the result is the same bit for bit on every run: each
+// wave takes 64 consecutive paths at a time and, vertex slot by vertex slot, groups its lanes by
+// material (the first remaining lane's material, then the next); a group's six contributions are
+// summed over the wave by a fixed butterfly (lanes outside the group add 0) and lane 0 adds the
+// sums to the wave's own row of per-material totals in LDS. No atomics are involved; the block
+// adds its waves' rows in wave order.
+constexpr uint32_t kAdjointBlock = 256;
+constexpr uint32_t kAdjointWaves = kAdjointBlock / 64;
+
+__global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams* __restrict__ dp, uint64_t P,
+                                                              const uint8_t* __restrict__ vmat,
+                                                              const uint8_t* __restrict__ vstat,
+                                                              const float* __restrict__ dimage, uint32_t M,
+                                                              float* __restrict__ partials)
 {
-    __shared__ float g[256 * 6]; // [material][dA rgb, dE rgb]
-    for (uint32_t k = threadIdx.x; k < 256 * 6; k += blockDim.x) g[k] = 0.0f;
+    __shared__ float g[kAdjointWaves][256 * 6]; // per wave: [material][dA rgb, dE rgb]
+    for (uint32_t k = threadIdx.x; k < kAdjointWaves * 256 * 6; k += blockDim.x) (&g[0][0])[k] = 0.0f;
     __syncthreads();
     const PtDevParams& d = *dp;
     const float inv_spp = 1.0f / float(d.spp);
+    const uint32_t lane = threadIdx.x & 63u;
+    float* const gw = g[threadIdx.x >> 6];
     const uint64_t stride_all = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < P; p += stride_all) {
-        const uint32_t stat = vstat[p];
+    // `base` is the same for the 64 lanes of a wave, so every lane runs every iteration
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); base < P; base += stride_all) {
+        const uint64_t p = base + lane;
+        const bool valid = p < P;
+        const uint32_t stat = valid ? vstat[p] : 0u;
         const int K = int(stat & ~kEscaped);
         const bool escaped = (stat & kEscaped) != 0;
-        const uint32_t pixel = uint32_t(p) / d.spp; // P < 2^32
+        const uint32_t pixel = valid ? uint32_t(p) / d.spp : 0u; // P < 2^32
         float delta[3];
-        for (int a = 0; a < 3; a++) delta[a] = dimage[uint64_t(pixel) * 3 + a] * inv_spp;
+        for (int a = 0; a < 3; a++) delta[a] = valid ? dimage[uint64_t(pixel) * 3 + a] * inv_spp : 0.0f;
         uint32_t vm[kMaxVertices];
         float vb[kMaxVertices][3];
         float b[3] = { 1.0f, 1.0f, 1.0f };
 #pragma unroll
         for (int j = 0; j < kMaxVertices; j++) {
+            vm[j] = 0u;
             if (j < K) {
                 const uint32_t m = vmat[uint64_t(j) * P + p];
                 vm[j] = m;
@@ -369,20 +385,38 @@ __global__ void __launch_bounds__(256) k_pt_adjoint(const PtDevParams* __restric
         for (int a = 0; a < 3; a++) G[a] = escaped ? d.sky[a] : 0.0f;
 #pragma unroll
         for (int j = kMaxVertices - 1; j >= 0; j--) {
-            if (j < K) {
-                const uint32_t m = vm[j];
+            const bool live = j < K;
+            const uint32_t m = vm[j];
+            float c[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+            if (live) {
                 for (int a = 0; a < 3; a++) {
                     const float w = delta[a] * vb[j][a];
-                    atomicAdd(&g[m * 6 + a], w * G[a]);
-                    atomicAdd(&g[m * 6 + 3 + a], w);
+                    c[a] = w * G[a];
+                    c[3 + a] = w;
                     G[a] = d.emission[m * 3 + a] + d.albedo[m * 3 + a] * G[a];
+                }
+            }
+            uint64_t todo = __ballot(live);
+            while (todo) {
+                const uint32_t m0 = uint32_t(__shfl(int(m), __ffsll((unsigned long long)todo) - 1));
+                const bool in = live && m == m0;
+                todo &= ~__ballot(in);
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    float v = in ? c[k] : 0.0f;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                    if (lane == 0) gw[m0 * 6 + k] += v;
                 }
             }
         }
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < M * 6; k += blockDim.x)
-        partials[uint64_t(blockIdx.x) * M * 6 + k] = g[6 + k]; // material ids 1..M
+    for (uint32_t k = threadIdx.x; k < M * 6; k += blockDim.x) { // material ids 1..M
+        float sum = 0.0f;
+        for (uint32_t w = 0; w < kAdjointWaves; w++) sum += g[w][6 + k];
+        partials[uint64_t(blockIdx.x) * M * 6 + k] = sum;
+    }
 }
 
 // One workgroup per output value ((material - 1) * 6 + component): strided FP64 sums over the
@@ -697,7 +731,7 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     st = get_plan(ctx, vol, pp, cfg, &p);
     if (st == CQX_OK) st = ensure_wavefront(ctx, p);
     if (st != CQX_OK) return st;
-    const uint32_t block = 256;
+    const uint32_t block = kAdjointBlock;
     const uint32_t blocks =
         uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(ctx->cu_count) * 8, (p->P + block - 1) / block)));
     const size_t need = size_t(blocks) * pp->n_materials * 6;

@@ -6,7 +6,8 @@ image (the image is a polynomial in the tables, so FD with h = 1e-3 is accurate)
 GPU: both renderers (hipGraph wavefront, megakernel) bit-identical to the FP32 oracle image
 (stated tolerance: RMSE <= 1e-6 of the mean radiance; observed 0); graph replays interleaved with
 eager work all equal; the adjoint within 1e-4 relative of the FP64 oracle gradient and within
-1e-3 relative of central finite differences (h = 1e-3) of the GPU loss.
+1e-3 relative of central finite differences (h = 1e-3) of the GPU loss, and bit-reproducible from
+call to call (its sums have a fixed order).
 """
 import numpy as np
 import pytest
@@ -178,6 +179,26 @@ def test_gpu_adjoint_vs_fp64_and_fd(cq, oracle, gpu_ctx, scene16):
     l2 = np.linalg.norm(gs - fds) / np.linalg.norm(fds)
     print(f"adjoint vs FD: max per-parameter {per.max():.2e}, L2 {l2:.2e}", flush=True)
     assert per.max() <= 1e-3 and l2 <= 1e-3, (per.max(), l2)
+    dv.free()
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_gpu_adjoint_is_bit_reproducible(cq, gpu_ctx, scene16):
+    """The adjoint's sums have a fixed order (no atomics): repeated calls, with other work on the
+    GPU in between, return the same gradient bit for bit."""
+    import torch
+
+    hv, s, _ = scene16
+    dv = gpu_ctx.upload(hv)
+    img = cq.pathtrace(gpu_ctx, dv, s, 0).double()
+    d_image = (2 * img - 0.25).float()
+    first = [t.cpu().numpy().copy() for t in cq.pathtrace_adjoint(gpu_ctx, dv, s, d_image)]
+    for k in range(3):
+        cq.pathtrace(gpu_ctx, dv, s.with_tables(s.albedo * 0.9, s.emission), 1)  # other work in between
+        again = [t.cpu().numpy() for t in cq.pathtrace_adjoint(gpu_ctx, dv, s, d_image)]
+        for a, b in zip(first, again):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"call {k + 2} differs"
     dv.free()
     torch.cuda.synchronize()
 
