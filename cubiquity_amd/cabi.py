@@ -139,6 +139,7 @@ SIGNATURES = {
     "cqx_volume_free": (_ST, [_P, _P]),
     "cqx_volume_info": (_ST, [_P, _P, _P, _P, _P]),
     "cqx_intersect": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
+    "cqx_intersect_noalloc": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_host": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P]),
     "cqx_intersect_batched": (_ST, [_P, _P, _U32, _P, _P]),
     "cqx_autotune": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P, _P]),

@@ -43,6 +43,7 @@ struct cqx_ctx {
     cqx::DemoPlan* demo_plan = nullptr;
     void* batch_desc = nullptr; // descriptor table of the batched launch
     size_t batch_bytes = 0;
+    void* noalloc_scratch = nullptr; // this device's static scratch (cqx_intersect_noalloc), not owned
     struct Tuned {
         int height; // SubDAG height of the volume tuned on
         int cls;    // batch-size class: floor(log2 n)
@@ -68,12 +69,22 @@ namespace cqx {
 // about n rays (or of the nearest tuned height and batch size, each within 2), or null.
 const cqx_launch_cfg* tuned_cfg(const cqx_ctx* ctx, int height, uint64_t n);
 
-// Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).
+// Device scratch a launch uses instead of the context's buffers: the persistent kernels' work
+// counters (1 KiB) and the ray-binning buffers (bins_bytes long).
+struct ScratchView {
+    void* counters = nullptr;
+    void* bins = nullptr;
+    size_t bins_bytes = 0;
+};
+
 // Grows the context's staging buffer (host-buffer entry points) to at least `bytes`.
 cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes);
+// Launches the traversal of `n` rays (or of *n_dev rays, at most n, when n_dev is not null).
+// Counters and binning buffers are the context's, or `sv`'s when it is given (then nothing is
+// allocated).
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                             const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
-                            const cqx_launch_cfg* cfg, hipStream_t stream);
+                            const cqx_launch_cfg* cfg, hipStream_t stream, const ScratchView* sv = nullptr);
 
 // Threads per block of the kernels that compact lists with block_slots.
 constexpr uint32_t kCompactBlock = 1024;

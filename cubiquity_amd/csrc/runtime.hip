@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -458,9 +459,11 @@ bool bin_grid(const cqx_dvolume* vol, uint32_t bits, BinGrid& g)
 }
 
 // Binning buffers of the context: kMaxCells counts, then n order entries (u32) and n cells (u16).
+size_t bin_bytes(uint64_t n);
+
 cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
 {
-    const size_t bytes = size_t(kMaxCells) * 4 + size_t(n) * 4 + size_t(n) * 2;
+    const size_t bytes = bin_bytes(n);
     if (bytes <= ctx->bins_bytes) return CQX_OK;
     HIP_TRY(hipDeviceSynchronize()); // launches in flight may use the old buffer
     release(ctx->bins);
@@ -472,13 +475,24 @@ cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
 }
 
 // Runs the binning prepass of n rays on `stream`; `order` receives the device pointer of the
-// resulting id -> ray map.
+// resulting id -> ray map. The buffers are the context's (grown as needed) or, with `sv`, the
+// caller-provided scratch, which must hold bin_bytes(n).
+size_t bin_bytes(uint64_t n) { return size_t(kMaxCells) * 4 + size_t(n) * 4 + size_t(n) * 2; }
+
 cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid& g, hipStream_t stream,
-                    const uint32_t*& order)
+                    const cqx::ScratchView* sv, const uint32_t*& order)
 {
-    cqx_status st = ensure_bins(ctx, n);
-    if (st != CQX_OK) return st;
-    uint32_t* counts = static_cast<uint32_t*>(ctx->bins);
+    void* bins = nullptr;
+    if (sv) {
+        if (bin_bytes(n) > sv->bins_bytes) return fail(CQX_ERR_LIMIT, "binning %llu rays needs %zu B of scratch, %zu B held",
+                                                      (unsigned long long)n, bin_bytes(n), sv->bins_bytes);
+        bins = sv->bins;
+    } else {
+        cqx_status st = ensure_bins(ctx, n);
+        if (st != CQX_OK) return st;
+        bins = ctx->bins;
+    }
+    uint32_t* counts = static_cast<uint32_t*>(bins);
     uint32_t* ord = counts + kMaxCells;
     uint16_t* cells = reinterpret_cast<uint16_t*>(ord + n);
     const uint32_t ncells = 1u << (3 * g.bits);
@@ -525,6 +539,42 @@ cqx_status check_ctx(cqx_ctx* ctx)
 
 } // namespace
 
+// Scratch of cqx_intersect_noalloc: the persistent kernels' work counters (8 queues, 128 B apart)
+// and the ray-binning buffers (cell counts, then the order and cells of at most
+// CQX_NOALLOC_MAX_BINNED_RAYS rays), in one file-scope device array. Every device the library
+// runs on holds its own copy until the process ends. Host code uses it only through the address
+// hipGetSymbolAddress returns with that device current (kept per device in noalloc_scratch_ptr),
+// never through the array's name, which on the host is a stand-in and not a device address.
+constexpr size_t kNoallocCounterBytes = 1024;
+constexpr size_t kNoallocBinBytes = size_t(kMaxCells) * 4 + size_t(CQX_NOALLOC_MAX_BINNED_RAYS) * 6;
+static_assert(kNoallocCounterBytes + kNoallocBinBytes == CQX_NOALLOC_SCRATCH_BYTES, "cqx.h states the scratch size");
+static_assert(sizeof(unsigned long long) * kQueues * kCounterStride <= kNoallocCounterBytes, "counters fit");
+static __device__ __align__(256) unsigned char scratch[kNoallocCounterBytes + kNoallocBinBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mutex;
+void* noalloc_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current), looked up once per device.
+cqx_status noalloc_scratch_address(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(CQX_ERR_LIMIT, "device %d beyond the %d supported", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (!noalloc_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(CQX_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+        noalloc_scratch_ptr[device] = p;
+    }
+    *out = noalloc_scratch_ptr[device];
+    return CQX_OK;
+}
+
+} // namespace
+
 namespace cqx {
 
 cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes)
@@ -541,8 +591,10 @@ cqx_status ensure_scratch(cqx_ctx* ctx, size_t bytes)
 
 cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                             const uint32_t* n_dev, uint32_t flags, float max_footprint, cqx_hit* hits,
-                            const cqx_launch_cfg* cfg, hipStream_t stream)
+                            const cqx_launch_cfg* cfg, hipStream_t stream, const ScratchView* sv)
 {
+    unsigned long long* const counters =
+        reinterpret_cast<unsigned long long*>(sv ? sv->counters : static_cast<void*>(ctx->d_counter));
     TraceParams P;
     P.nodes = vol->nodes;
     P.sub = vol->sub;
@@ -569,7 +621,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         if (n > 0xffffffffull) return fail(CQX_ERR_LIMIT, "ray binning takes fewer than 2^32 rays");
         BinGrid g;
         if (bin_grid(vol, bin_bits, g)) {
-            const cqx_status bst = bin_rays(ctx, rays, n, g, stream, order);
+            const cqx_status bst = bin_rays(ctx, rays, n, g, stream, sv, order);
             if (bst != CQX_OK) return bst;
         }
     }
@@ -585,10 +637,9 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         if (lds2 > size_t(ctx->lds_per_block))
             return fail(CQX_ERR_LIMIT, "two stacks per lane need %zu B of LDS per block", lds2);
         SingleSource src{ P, rays, hits, order };
-        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
-        hipLaunchKernelGGL(k_intersect_pair, dim3(blocks), dim3(block), lds2, stream, src, n, n_dev,
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter), regroup, refill, rows, chunk);
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
+        hipLaunchKernelGGL(k_intersect_pair, dim3(blocks), dim3(block), lds2, stream, src, n, n_dev, counters, regroup,
+                           refill, rows, chunk);
     } else if (variant >= 2 && variant <= 4) {
         uint32_t blocks, regroup, refill, chunk;
         cqx_status st = persistent_shape(ctx, cfg, block, n, blocks, regroup, refill, chunk);
@@ -603,9 +654,7 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
                 return fail(CQX_ERR_LIMIT, "stack and %u LDS nodes need %zu B of LDS per block", top_nodes, lds_total);
         }
         SingleSource src{ P, rays, hits, order };
-        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream,
-                           reinterpret_cast<unsigned long long*>(ctx->d_counter));
-        unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
+        hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
         if (variant == 2)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
                                n, n_dev, counters, regroup, refill, rows, 0u, chunk);
@@ -649,6 +698,13 @@ cqx_status cqx_create(int device, cqx_ctx** out)
         release(c->d_counter);
         delete c;
         return fail(CQX_ERR_HIP, "stream creation failed");
+    }
+    const cqx_status st = noalloc_scratch_address(device, &c->noalloc_scratch);
+    if (st != CQX_OK) {
+        (void)hipStreamDestroy(c->capture_stream);
+        release(c->d_counter);
+        delete c;
+        return st;
     }
     *out = c;
     return CQX_OK;
@@ -827,6 +883,37 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
         }
     }
     return cqx::launch_intersect(ctx, vol, rays, n, nullptr, flags, max_footprint, hits, cfg, hipStream_t(stream));
+}
+
+cqx_status cqx_intersect_noalloc(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
+                                 float max_footprint, cqx_hit* hits, const cqx_launch_cfg* cfg, void* stream)
+{
+    // Allocates nothing: validation, the tuned-launch lookup and kernel launches only; the
+    // counters and binning buffers are the device's static scratch (cleared by the launches).
+    cqx_status st = check_ctx(ctx);
+    if (st != CQX_OK) return st;
+    if (!vol) return fail(CQX_ERR_INVALID_ARG, "null volume");
+    if (flags & ~CQX_SURFACE_PROPERTIES) return fail(CQX_ERR_INVALID_ARG, "unknown flags 0x%x", flags);
+    if (n == 0) return CQX_OK;
+    if (!rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null ray or hit buffer");
+    if (n > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "%llu rays exceed the 2^38 limit", (unsigned long long)n);
+    if (!ctx->noalloc_scratch) return fail(CQX_ERR_HIP, "static scratch of device %d not set up", ctx->device);
+    cqx_launch_cfg eff;
+    if (!cfg || cfg->variant == 0) {
+        if (const cqx_launch_cfg* t = cqx::tuned_cfg(ctx, vol->max_height, n)) {
+            eff = *t;
+            if (cfg && cfg->max_iters) eff.max_iters = cfg->max_iters;
+            cfg = &eff;
+        }
+    }
+    if (cfg && cfg->bin_bits && n > CQX_NOALLOC_MAX_BINNED_RAYS)
+        return fail(CQX_ERR_LIMIT, "cqx_intersect_noalloc bins at most %u rays (%llu given); use cqx_intersect",
+                    (unsigned)CQX_NOALLOC_MAX_BINNED_RAYS, (unsigned long long)n);
+    cqx::ScratchView sv;
+    sv.counters = ctx->noalloc_scratch;
+    sv.bins = static_cast<unsigned char*>(ctx->noalloc_scratch) + kNoallocCounterBytes;
+    sv.bins_bytes = kNoallocBinBytes;
+    return cqx::launch_intersect(ctx, vol, rays, n, nullptr, flags, max_footprint, hits, cfg, hipStream_t(stream), &sv);
 }
 
 cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_t n_batches, const cqx_launch_cfg* cfg,

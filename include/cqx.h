@@ -312,6 +312,20 @@ typedef struct {
 cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                          uint32_t flags, float max_footprint, cqx_hit* hits,
                          const cqx_launch_cfg* cfg, void* stream);
+/* cqx_intersect that allocates nothing (no counterpart in the reference: for callers that cannot
+ * allocate device memory, such as code under HIP graph capture, tight serving loops and harnesses
+ * that forbid hipMalloc). Same arguments, semantics and records as cqx_intersect. Instead of the
+ * context's buffers, the work counters and the ray-binning buffers live in one static device
+ * array per GPU (CQX_NOALLOC_SCRATCH_BYTES, held from cqx_create until the process ends), which
+ * every call on that GPU shares: each call clears what must start at zero, and calls that overlap
+ * on one GPU (any stream, any context) race. Ray binning (cfg->bin_bits > 0, also when the tuned
+ * "auto" launch bins) takes at most CQX_NOALLOC_MAX_BINNED_RAYS rays; more fail with
+ * CQX_ERR_LIMIT and nothing is launched. */
+#define CQX_NOALLOC_MAX_BINNED_RAYS (1u << 22)
+#define CQX_NOALLOC_SCRATCH_BYTES (1024u + 4096u * 4u + 6u * CQX_NOALLOC_MAX_BINNED_RAYS)
+cqx_status cqx_intersect_noalloc(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
+                                 uint32_t flags, float max_footprint, cqx_hit* hits,
+                                 const cqx_launch_cfg* cfg, void* stream);
 /* Launch autotuner (no reference counterpart). Times kernel variant, rays per workgroup,
  * persistent waves per CU, regroup interval and refill threshold on these rays (device pointers;
  * `hits` is overwritten), picks the fastest by coordinate descent and caches it in the context
