@@ -353,11 +353,7 @@ struct EditWalk {
 // list (RegionStore), in parallel; the lists are merged into the store as if interned in region
 // order (merge_regions), so the result does not depend on the number of threads.
 
-struct RegionKey {
-    uint32_t x, y, z; // region index along each axis of the offset coordinates u = v + 2^31
-    bool operator<(const RegionKey& o) const { return x != o.x ? x < o.x : y != o.y ? y < o.y : z < o.z; }
-    bool operator==(const RegionKey& o) const { return x == o.x && y == o.y && z == o.z; }
-};
+} // namespace
 
 // Smallest region height for these edits (>= 5), or -1 for sequential edits: spheres too large,
 // or bounds not within 2^23 voxels of the origin (there every voxel coordinate the walk converts
@@ -378,10 +374,78 @@ int region_height(const cqx_sphere_edit* edits, uint64_t n)
     return -1;
 }
 
+bool morton_less(const RegionKey& a, const RegionKey& b)
+{
+    // the axis whose differing bit is the most significant; at equal bits z ranks above y above x
+    const uint32_t d[3] = { a.x ^ b.x, a.y ^ b.y, a.z ^ b.z };
+    const auto below = [](uint32_t p, uint32_t q) { return p < q && p < (p ^ q); }; // msb(p) < msb(q)
+    int axis = 2;
+    if (below(d[axis], d[1])) axis = 1;
+    if (below(d[axis], d[0])) axis = 0;
+    const uint32_t av = axis == 0 ? a.x : axis == 1 ? a.y : a.z, bv = axis == 0 ? b.x : axis == 1 ? b.y : b.z;
+    return av < bv;
+}
+
+void plan_edits(const cqx_sphere_edit* edits, uint64_t n, int hs, EditPlan& plan)
+{
+    plan = EditPlan{};
+    plan.hs = hs;
+    // Regions each edit's bounds touch (one voxel of margin), edits kept in order per region.
+    std::vector<std::pair<RegionKey, uint32_t>> touch;
+    for (uint64_t i = 0; i < n; i++) {
+        uint32_t lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            const float c = edits[i].centre[a], r = edits[i].radius;
+            const float blo = c - r, bhi = c + r; // SphereBrush bounds, FP32
+            const int64_t vlo = int64_t(std::floor(double(blo))) - 1, vhi = int64_t(std::ceil(double(bhi))) + 1;
+            lo[a] = uint32_t(std::max<int64_t>(vlo, INT32_MIN) + (int64_t(1) << 31)) >> hs;
+            hi[a] = uint32_t(std::min<int64_t>(vhi, INT32_MAX) + (int64_t(1) << 31)) >> hs;
+        }
+        for (uint32_t z = lo[2]; z <= hi[2]; z++)
+            for (uint32_t y = lo[1]; y <= hi[1]; y++)
+                for (uint32_t x = lo[0]; x <= hi[0]; x++) touch.push_back({ RegionKey{ x, y, z }, uint32_t(i) });
+    }
+    std::stable_sort(touch.begin(), touch.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    plan.edit_ids.resize(touch.size());
+    for (size_t j = 0; j < touch.size(); j++) {
+        plan.edit_ids[j] = touch[j].second;
+        if (j == 0 || !(touch[j].first == touch[j - 1].first)) {
+            plan.keys.push_back(touch[j].first);
+            plan.first.push_back(j);
+        }
+    }
+    plan.first.push_back(touch.size());
+    const size_t nr = plan.keys.size();
+    plan.morton_regions.resize(nr);
+    for (size_t r = 0; r < nr; r++) plan.morton_regions[r] = uint32_t(r);
+    std::sort(plan.morton_regions.begin(), plan.morton_regions.end(),
+              [&](uint32_t a, uint32_t b) { return morton_less(plan.keys[a], plan.keys[b]); });
+    const int levels = kRootHeight - hs;
+    plan.level_keys.assign(size_t(levels) + 1, {});
+    plan.level_children.assign(size_t(levels) + 1, {});
+    for (size_t r = 0; r < nr; r++) plan.level_keys[0].push_back(plan.keys[plan.morton_regions[r]]);
+    for (int l = 1; l <= levels; l++) {
+        const std::vector<RegionKey>& below = plan.level_keys[size_t(l) - 1];
+        std::vector<RegionKey>& up = plan.level_keys[size_t(l)];
+        std::vector<uint32_t>& ch = plan.level_children[size_t(l)];
+        for (size_t j = 0; j < below.size(); j++) {
+            const RegionKey k{ below[j].x >> 1, below[j].y >> 1, below[j].z >> 1 };
+            if (up.empty() || !(up.back() == k)) {
+                up.push_back(k);
+                ch.push_back(uint32_t(j));
+            }
+        }
+        ch.push_back(uint32_t(below.size()));
+    }
+}
+
+namespace {
+
 // The node at height hs whose region is `k`, found by descending from the root (Volume::voxel's
 // walk, storage.cpp:481-499).
 uint32_t region_node(const std::vector<Node>& nodes, uint32_t root, int hs, const RegionKey& k)
 {
+    if (hs >= kRootHeight) return root;
     uint32_t node = root;
     const uint32_t ux = k.x << hs, uy = k.y << hs, uz = k.z << hs;
     for (int h = kRootHeight; h > hs && node >= kMaterialCount; h--) {
@@ -513,29 +577,37 @@ void merge_regions(std::vector<Node>& nodes, Interner& in, uint32_t base, std::v
     in.index_appended();
 }
 
-// Rebuilds the levels above the regions: `regs` indexes the edited regions inside this node.
-uint32_t rebuild_top(GlobalStore& g, uint32_t node, int h, int hs, const std::vector<RegionKey>& keys,
-                     const std::vector<uint32_t>& results, const std::vector<uint32_t>& regs)
+// Rebuilds the levels above the regions, bottom-up: at each height hs + l the nodes that contain
+// edited regions, in Morton order (plan.level_keys), take their changed children and the changed
+// ones are interned in that order. `results` holds each region's new node (by region index).
+uint32_t rebuild_levels(GlobalStore& g, uint32_t root, const EditPlan& plan, const std::vector<uint32_t>& results)
 {
-    if (h == hs) return results[regs[0]];
-    std::vector<uint32_t> bucket[8];
-    for (uint32_t r : regs) {
-        const uint32_t ux = keys[r].x << hs, uy = keys[r].y << hs, uz = keys[r].z << hs;
-        const uint32_t c = ((ux >> (h - 1)) & 1u) | (((uy >> (h - 1)) & 1u) << 1) | (((uz >> (h - 1)) & 1u) << 2);
-        bucket[c].push_back(r);
-    }
-    Node out;
-    for (int c = 0; c < 8; c++) out.c[c] = g.child(node, uint32_t(c));
-    bool changed = false;
-    for (int c = 0; c < 8; c++) {
-        if (bucket[c].empty()) continue;
-        const uint32_t nc = rebuild_top(g, out.c[c], h - 1, hs, keys, results, bucket[c]);
-        if (nc != out.c[c]) {
-            out.c[c] = nc;
-            changed = true;
+    std::vector<uint32_t> cur(plan.morton_regions.size()), next;
+    for (size_t i = 0; i < cur.size(); i++) cur[i] = results[plan.morton_regions[i]];
+    const int levels = kRootHeight - plan.hs;
+    for (int l = 1; l <= levels; l++) {
+        const int h = plan.hs + l;
+        const std::vector<RegionKey>& keys = plan.level_keys[size_t(l)];
+        const std::vector<RegionKey>& below = plan.level_keys[size_t(l) - 1];
+        const std::vector<uint32_t>& ch = plan.level_children[size_t(l)];
+        next.assign(keys.size(), 0);
+        for (size_t i = 0; i < keys.size(); i++) {
+            const uint32_t old = region_node(g.nodes, root, h, keys[i]);
+            Node out;
+            for (int c = 0; c < 8; c++) out.c[c] = g.child(old, uint32_t(c));
+            bool changed = false;
+            for (uint32_t j = ch[i]; j < ch[i + 1]; j++) {
+                const uint32_t id = (below[j].x & 1u) | (below[j].y & 1u) << 1 | (below[j].z & 1u) << 2;
+                if (cur[j] != out.c[id]) {
+                    out.c[id] = cur[j];
+                    changed = true;
+                }
+            }
+            next[i] = changed ? g.intern(out) : old;
         }
+        cur.swap(next);
     }
-    return changed ? g.intern(out) : node;
+    return cur[0];
 }
 
 } // namespace
@@ -577,30 +649,10 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
             return std::chrono::duration<double, std::milli>(clock::now() - t).count();
         };
         const clock::time_point t_start = clock::now();
-        // Regions each edit's bounds touch (one voxel of margin), edits kept in order per region.
-        std::vector<std::pair<RegionKey, uint32_t>> touch;
-        for (uint64_t i = 0; i < n; i++) {
-            uint32_t lo[3], hi[3];
-            for (int a = 0; a < 3; a++) {
-                const SphereBrushF b(edits[i]);
-                const int64_t vlo = int64_t(std::floor(double(b.lo[a]))) - 1, vhi = int64_t(std::ceil(double(b.hi[a]))) + 1;
-                lo[a] = uint32_t(std::max<int64_t>(vlo, INT32_MIN) + (int64_t(1) << 31)) >> hs;
-                hi[a] = uint32_t(std::min<int64_t>(vhi, INT32_MAX) + (int64_t(1) << 31)) >> hs;
-            }
-            for (uint32_t z = lo[2]; z <= hi[2]; z++)
-                for (uint32_t y = lo[1]; y <= hi[1]; y++)
-                    for (uint32_t x = lo[0]; x <= hi[0]; x++) touch.push_back({ RegionKey{ x, y, z }, uint32_t(i) });
-        }
-        std::stable_sort(touch.begin(), touch.end(),
-                         [](const auto& a, const auto& b) { return a.first < b.first; });
-        std::vector<RegionKey> keys;
-        std::vector<size_t> first; // first entry of each region in `touch`
-        for (size_t j = 0; j < touch.size(); j++)
-            if (j == 0 || !(touch[j].first == touch[j - 1].first)) {
-                keys.push_back(touch[j].first);
-                first.push_back(j);
-            }
-        first.push_back(touch.size());
+        EditPlan plan;
+        plan_edits(edits, n, hs, plan);
+        const std::vector<RegionKey>& keys = plan.keys;
+        const std::vector<uint64_t>& first = plan.first;
         const size_t nr = keys.size();
         const uint32_t base = uint32_t(nodes.size());
         std::vector<RegionStore> stores;
@@ -616,8 +668,8 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
                     uint32_t node = region_node(nodes, hv->v.root, hs, k);
                     const int32_t lx = int32_t((k.x << hs) - 0x80000000u), ly = int32_t((k.y << hs) - 0x80000000u),
                                   lz = int32_t((k.z << hs) - 0x80000000u);
-                    for (size_t j = first[r]; j < first[r + 1]; j++) {
-                        const cqx_sphere_edit& e = edits[touch[j].second];
+                    for (uint64_t j = first[r]; j < first[r + 1]; j++) {
+                        const cqx_sphere_edit& e = edits[plan.edit_ids[j]];
                         const SphereBrushF b(e);
                         EditWalk<RegionStore> w{ stores[r], b, e.material };
                         node = w.fill(node, hs, lx, ly, lz);
@@ -643,9 +695,7 @@ cqx_status cqx_hv_edit_spheres_mt(cqx_host_volume* hv, const cqx_sphere_edit* ed
         stores.clear();
         const double merge_ms = ms_since(t_merge);
         const clock::time_point t_top = clock::now();
-        std::vector<uint32_t> all(nr);
-        for (size_t r = 0; r < nr; r++) all[r] = uint32_t(r);
-        hv->v.root = rebuild_top(g, hv->v.root, kRootHeight, hs, keys, results, all);
+        hv->v.root = rebuild_levels(g, hv->v.root, plan, results);
         log_debug("edits: %llu spheres over %zu regions of side 2^%d on %d threads; plan %.1f ms, apply %.1f ms, "
                   "merge %.1f ms, top levels %.1f ms; %zu nodes appended",
                   (unsigned long long)n, nr, hs, nt, plan_ms, apply_ms, merge_ms, ms_since(t_top),
