@@ -68,6 +68,18 @@ def gen_default_params(log2_size, seed, **overrides):
     return p
 
 
+def _sphere_edits(edits):
+    """(n, 5) centre x, y, z, radius, material -> cqx_sphere_edit records (FP32 centre and radius)."""
+    e = np.asarray(edits, dtype=np.float64).reshape(-1, 5)
+    arr = np.zeros(len(e), dtype=SPHERE_EDIT_DTYPE)
+    arr["centre"] = e[:, :3]
+    arr["radius"] = e[:, 3]
+    if len(e) and not (np.all(e[:, 4] >= 0) and np.all(e[:, 4] < 2**32)):
+        raise ValueError("material out of range")
+    arr["material"] = e[:, 4].astype(np.uint64)
+    return arr
+
+
 class HostVolume:
     """Host node store + root: the library's view of a reference ``Volume``."""
 
@@ -113,15 +125,9 @@ class HostVolume:
         """Sphere fill/carve edits (fillBrush with SphereBrush semantics). `edits`: (n, 5) array of
         centre x, y, z, radius, material (0 carves); `threads`: 0 all cores, 1 in order on this
         thread. Returns the index of the first appended node."""
-        e = np.asarray(edits, dtype=np.float64).reshape(-1, 5)
-        arr = np.zeros(max(1, len(e)), dtype=SPHERE_EDIT_DTYPE)
-        arr["centre"][: len(e)] = e[:, :3]
-        arr["radius"][: len(e)] = e[:, 3]
-        if len(e) and not (np.all(e[:, 4] >= 0) and np.all(e[:, 4] < 2**32)):
-            raise CqxError("material out of range")
-        arr["material"][: len(e)] = e[:, 4].astype(np.uint64)
+        arr = _sphere_edits(edits)
         first = ctypes.c_uint64()
-        check(lib().cqx_hv_edit_spheres_mt(self._h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(e)),
+        check(lib().cqx_hv_edit_spheres_mt(self._h, arr.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(len(arr)),
                                            ctypes.c_int32(threads), ctypes.byref(first)))
         return int(first.value)
 
@@ -241,6 +247,43 @@ class DeviceVolume:
                                       ctypes.c_uint32(volume.root), None))
         self.node_count = nodes.shape[0]
         self.root = volume.root
+
+    def edit_spheres(self, edits):
+        """Sphere fill/carve edits on the device copy (cqx_volume_edit_spheres): the store
+        HostVolume.edit_spheres(edits, threads > 1) gives, computed on the GPU. `edits`: (n, 5)
+        centre x, y, z, radius, material. Returns the index of the first appended node."""
+        arr = _sphere_edits(edits)
+        first = ctypes.c_uint64()
+        check(lib().cqx_volume_edit_spheres(self._ctx._h, self._h, arr.ctypes.data_as(ctypes.c_void_p),
+                                            ctypes.c_uint64(len(arr)), ctypes.byref(first)))
+        self._refresh()
+        return int(first.value)
+
+    def checkpoint(self):
+        check(lib().cqx_volume_checkpoint(self._ctx._h, self._h))
+
+    def undo(self):
+        check(lib().cqx_volume_undo(self._ctx._h, self._h))
+        self._refresh()
+
+    def redo(self):
+        check(lib().cqx_volume_redo(self._ctx._h, self._h))
+        self._refresh()
+
+    def download(self, first=0, count=None):
+        """Nodes [first, first + count) of the device store as a (count, 8) uint32 array."""
+        if count is None:
+            count = self.node_count - first
+        out = np.zeros((count, 8), np.uint32)
+        check(lib().cqx_volume_download(self._ctx._h, self._h, ctypes.c_uint64(first), ctypes.c_uint64(count),
+                                        out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def _refresh(self):
+        n = ctypes.c_uint64()
+        r = ctypes.c_uint32()
+        check(lib().cqx_volume_info(self._h, None, ctypes.byref(n), ctypes.byref(r), None))
+        self.node_count, self.root = int(n.value), int(r.value)
 
     def info(self):
         nodes = ctypes.c_void_p()

@@ -138,6 +138,11 @@ SIGNATURES = {
     "cqx_volume_upload": (_ST, [_P, ctypes.POINTER(ctypes.c_uint32), _U64, _U32, _P, _P]),
     "cqx_volume_free": (_ST, [_P, _P]),
     "cqx_volume_info": (_ST, [_P, _P, _P, _P, _P]),
+    "cqx_volume_edit_spheres": (_ST, [_P, _P, _P, _U64, _P]),
+    "cqx_volume_checkpoint": (_ST, [_P, _P]),
+    "cqx_volume_undo": (_ST, [_P, _P]),
+    "cqx_volume_redo": (_ST, [_P, _P]),
+    "cqx_volume_download": (_ST, [_P, _P, _U64, _U64, _P]),
     "cqx_intersect": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_noalloc": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P, _P, _P]),
     "cqx_intersect_host": (_ST, [_P, _P, _P, _U64, _U32, ctypes.c_float, _P]),

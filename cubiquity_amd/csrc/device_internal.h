@@ -21,6 +21,8 @@ struct PtPlan; // path-tracer buffers and captured graph (pathtrace.hip)
 void destroy_pt_plan(PtPlan* plan);
 struct DemoPlan; // buffers of the reference-shading mode (demo.hip)
 void destroy_demo_plan(DemoPlan* plan);
+struct DeviceEdits; // a volume's on-device edit state (edits_gpu.hip)
+void destroy_device_edits(DeviceEdits* e);
 
 // Best-effort release paths: an error here has nowhere useful to go.
 inline void release(void* p) { (void)hipFree(p); }
@@ -61,6 +63,7 @@ struct cqx_dvolume {
     uint32_t root = 0;
     cqx_subdag hsub[8];
     int max_height = 0; // largest SubDAG start height (stack rows needed: max_height + 1)
+    cqx::DeviceEdits* edits = nullptr; // index and buffers of on-device edits, made on first use
 };
 
 namespace cqx {

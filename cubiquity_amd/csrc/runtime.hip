@@ -848,6 +848,7 @@ cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol)
     }
     release(vol->nodes);
     release(vol->sub);
+    cqx::destroy_device_edits(vol->edits);
     delete vol;
     return CQX_OK;
 }

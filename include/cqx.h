@@ -280,6 +280,27 @@ cqx_status cqx_volume_free(cqx_ctx* ctx, cqx_dvolume* vol);
  * gpu_pathtracing_viewer.h:14-26.) */
 cqx_status cqx_volume_update(cqx_ctx* ctx, cqx_dvolume* vol, const uint32_t* nodes, uint64_t node_count,
                              uint64_t first, uint32_t root, const cqx_subdag sub[8]);
+/* Sphere fill / carve edits applied on the device to the device copy of the store
+ * (fillBrush(volume, SphereBrush(centre, radius), material) in order, voxelization.cpp:825-915,
+ * NodeStore::setNodeChild's copy-on-write, storage.cpp:152-167; the reference has no device
+ * edits). The result is cqx_hv_edit_spheres_mt's with more than one thread, node for node: the
+ * same region-parallel algorithm, hash-consing against an index over the device store (built on
+ * the volume's first edit, 4 bytes x 2-4 x node_count of device memory, kept up to date), new
+ * nodes appended from *first_new on; the root and SubDAGs are replaced. Batches of fewer than 16
+ * spheres, spheres farther than 2^23 voxels from the origin and radii too large for a region of
+ * side 2^20 are refused with CQX_ERR_LIMIT (apply those with cqx_hv_edit_spheres). Synchronizes
+ * the device. */
+cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sphere_edit* edits, uint64_t n,
+                                   uint64_t* first_new);
+/* Edit history of a device volume (Volume::checkpoint / undo / redo, storage.cpp:358-385): roots
+ * recorded and switched; no node changes, the SubDAGs are recomputed on the device. */
+cqx_status cqx_volume_checkpoint(cqx_ctx* ctx, cqx_dvolume* vol);
+cqx_status cqx_volume_undo(cqx_ctx* ctx, cqx_dvolume* vol);
+cqx_status cqx_volume_redo(cqx_ctx* ctx, cqx_dvolume* vol);
+/* Copies nodes [first, first + count) of the device store to host memory (node x 8 uint32),
+ * e.g. the nodes cqx_volume_edit_spheres appended, to save the volume. */
+cqx_status cqx_volume_download(cqx_ctx* ctx, const cqx_dvolume* vol, uint64_t first, uint64_t count,
+                               uint32_t* nodes);
 /* Device pointer to the volume's node array (node_count x 8 uint32) and its node count. */
 cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, uint64_t* node_count,
                            uint32_t* root, cqx_subdag sub[8]);
