@@ -3,10 +3,10 @@
 //
 //   planning (host, plan_edits): the regions of side 2^hs each sphere's bounds touch, with their
 //     spheres in order, and the levels above the regions in Morton order;
-//   k_edit_regions: one lane per region descends to the region's node and applies its spheres in
+//   k_edit_regions: one wave per region descends to the region's node and applies its spheres in
 //     order with fillBrush's arithmetic (voxelization.cpp:825-915; EditWalk::fill, here with an
-//     explicit stack), interning changed nodes into the region's own list with its own hash
-//     table; a node without listed children is first looked up in the device index over the
+//     explicit stack and each visited node's 8 children tested on the wave's 64 lanes at once),
+//     interning changed nodes into the region's own list with its own hash table; a node without listed children is first looked up in the device index over the
 //     store. A list that outgrows its slice stops its region, which runs again with a larger one;
 //   merge (k_merge_*): the lists laid end to end in region order and merged as merge_regions does:
 //     level by level, the lowest list position among equal nodes is kept, the kept ones are
@@ -21,7 +21,9 @@
 #include "device_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -236,7 +238,7 @@ struct Region {
         uint32_t* const tab = A.table + poff * 2;
         for (uint64_t i = hash_node(n) & mask;; i = (i + 1) & mask) {
             const uint32_t s = tab[i];
-            if (s == kEmptySlot) {
+            if (s == kEmptySlot) { // every lane of the wave stores the same values
                 if (count >= cap) return kOverflow;
                 store_node(A.pool, poff + count, n);
                 tab[i] = count;
@@ -247,106 +249,136 @@ struct Region {
     }
 };
 
+// One wave walks one region: the DFS of EditWalk::fill (the same children in the same order, the
+// same nodes interned in the same order) runs in lockstep on all 64 lanes, and at each node entered
+// the lanes evaluate its 8 children at once: lane l takes child l / 8 and corner l % 8 of that
+// child's box, so the 64 corner tests of SphereBrush::contains, the 8 overlap tests, the 8
+// box_outside tests and the 8 child reads are one step. A frame keeps the node's children and
+// the per-child results as 8-bit masks, wave-uniform, in LDS.
 struct Frame {
+    uint32_t out[8];
     uint32_t node;
-    DNode out;
     int32_t lo[3];
-    uint8_t ok[3];
-    uint8_t next;
-    uint8_t changed;
-    uint8_t height;
+    uint32_t height;
+    uint32_t next;    // next child position to consider
+    uint32_t changed;
+    uint32_t ok;      // children whose float box overlaps the brush bounds
+    uint32_t outside; // children wholly outside the sphere (box_outside)
+    uint32_t inside;  // children with all 8 corners inside
 };
 
-// fillBrush's descent into `node` (height >= 1, lower corner lo): which children the brush bounds
-// overlap (geometry.h:346-357, float boxes of the integer corners, two tests per axis); false
-// when none does, and the node comes back unchanged.
-__device__ bool enter(const Region& R, const Brush& b, Frame& f, uint32_t node, int height, int32_t lx, int32_t ly,
-                      int32_t lz)
+// Bit c (c = 0..7) set when lane 8c's bit of `ballot` is set.
+__device__ __forceinline__ uint32_t group_bits(uint64_t ballot)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) m |= uint32_t((ballot >> (8 * c)) & 1u) << c;
+    return m;
+}
+
+// fillBrush's descent into `node` (height >= 1, lower corner lo); false when no child's float box
+// overlaps the brush bounds (geometry.h:346-357) and the node comes back unchanged.
+__device__ bool enter(const Region& R, const Brush& b, Frame& f, uint32_t node, uint32_t height, int32_t lx, int32_t ly,
+                      int32_t lz, uint32_t lane)
 {
     const uint32_t side = 1u << (height - 1);
-    const int32_t lo[3] = { lx, ly, lz };
-    uint8_t ok[3] = { 0, 0, 0 };
-    for (int a = 0; a < 3; a++)
-        for (uint32_t p = 0; p <= 1; p++) {
-            const int32_t v0 = int32_t(uint32_t(lo[a]) + side * p);
-            const int32_t v1 = int32_t(uint32_t(v0) + (side - 1));
-            if (!(b.hi[a] < float(v0) || b.lo[a] > float(v1))) ok[a] |= uint8_t(1u << p);
-        }
-    if (!(ok[0] && ok[1] && ok[2])) return false;
-    f.node = node;
-    f.height = uint8_t(height);
-    for (int a = 0; a < 3; a++) {
-        f.lo[a] = lo[a];
-        f.ok[a] = ok[a];
+    const uint32_t c = lane >> 3, k = lane & 7u;
+    const uint32_t cx = c & 1u, cy = (c >> 1) & 1u, cz = c >> 2;
+    const int32_t x0 = int32_t(uint32_t(lx) + side * cx), y0 = int32_t(uint32_t(ly) + side * cy),
+                  z0 = int32_t(uint32_t(lz) + side * cz);
+    const float fx0 = float(x0), fy0 = float(y0), fz0 = float(z0);
+    const float fx1 = float(int32_t(uint32_t(x0) + (side - 1))), fy1 = float(int32_t(uint32_t(y0) + (side - 1))),
+                fz1 = float(int32_t(uint32_t(z0) + (side - 1)));
+    const bool ok = !(b.hi[0] < fx0 || b.lo[0] > fx1) && !(b.hi[1] < fy0 || b.lo[1] > fy1) &&
+                    !(b.hi[2] < fz0 || b.lo[2] > fz1);
+    const uint32_t okm = group_bits(__ballot(ok));
+    if (okm == 0) return false;
+    // a single voxel's 8 corners are one point
+    const bool in = b.contains((k & 1u) ? fx1 : fx0, (k & 2u) ? fy1 : fy0, (k & 4u) ? fz1 : fz0);
+    const uint64_t inb = __ballot(in);
+    uint32_t allm = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) allm |= (((inb >> (8 * j)) & 0xffull) == 0xffull ? 1u : 0u) << j;
+    const bool out = height >= 2 && b.box_outside(fx0, fy0, fz0, fx1, fy1, fz1);
+    const uint32_t outm = group_bits(__ballot(out));
+    const uint32_t child = R.child(node, c);
+    if (k == 0) f.out[c] = child;
+    if (lane == 0) {
+        f.node = node;
+        f.height = height;
+        f.lo[0] = lx;
+        f.lo[1] = ly;
+        f.lo[2] = lz;
+        f.next = 0;
+        f.changed = 0;
+        f.ok = okm;
+        f.outside = outm;
+        f.inside = allm;
     }
-    f.next = 0;
-    f.changed = 0;
-    for (uint32_t c = 0; c < 8; c++) f.out.c[c] = R.child(node, c);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     return true;
 }
 
-// EditWalk::fill (edits.cpp) without recursion: the same children in the same order, the same
-// tests, the same nodes interned in the same order. Returns the new node, or kOverflow.
-__device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0, int height0, int32_t lx, int32_t ly,
-                         int32_t lz)
+// EditWalk::fill (edits.cpp) without recursion, on the whole wave; returns the new node or
+// kOverflow (the same value in every lane).
+__device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0, uint32_t height0, int32_t lx,
+                         int32_t ly, int32_t lz, Frame* st, uint32_t lane)
 {
-    Frame st[kMaxRegionHeight + 1];
     int sp = 0;
-    if (!enter(R, b, st[0], node0, height0, lx, ly, lz)) return node0;
+    if (!enter(R, b, st[0], node0, height0, lx, ly, lz, lane)) return node0;
     bool returned = false;
     uint32_t ret = 0;
     for (;;) {
         Frame& f = st[sp];
-        if (returned) { // the child frame of position f.next came back with `ret`
+        uint32_t next = f.next, changed = f.changed;
+        if (returned) { // the child frame of position `next` came back with `ret`
             if (ret == kOverflow) return kOverflow;
-            if (ret != f.out.c[f.next]) {
-                f.out.c[f.next] = ret;
-                f.changed = 1;
+            if (ret != f.out[next]) {
+                if (lane == 0) f.out[next] = ret;
+                changed = 1;
             }
-            f.next++;
+            next++;
             returned = false;
         }
         bool pushed = false;
         const uint32_t side = 1u << (f.height - 1);
-        while (f.next < 8) {
-            const uint32_t id = f.next, cx = id & 1u, cy = (id >> 1) & 1u, cz = id >> 2;
-            if (!((f.ok[0] >> cx) & (f.ok[1] >> cy) & (f.ok[2] >> cz) & 1u)) {
-                f.next++;
-                continue;
-            }
-            const uint32_t child = f.out.c[id];
-            if (child == mat) {
-                f.next++;
-                continue;
-            }
-            const int32_t x0 = int32_t(uint32_t(f.lo[0]) + side * cx), y0 = int32_t(uint32_t(f.lo[1]) + side * cy),
-                          z0 = int32_t(uint32_t(f.lo[2]) + side * cz);
-            const float fx0 = float(x0), fy0 = float(y0), fz0 = float(z0);
-            const float fx1 = float(int32_t(uint32_t(x0) + (side - 1))), fy1 = float(int32_t(uint32_t(y0) + (side - 1))),
-                        fz1 = float(int32_t(uint32_t(z0) + (side - 1)));
-            if (f.height >= 2 && b.box_outside(fx0, fy0, fz0, fx1, fy1, fz1)) {
-                f.next++;
-                continue;
-            }
-            const bool all = side == 1u ? b.contains(fx0, fy0, fz0)
-                                        : b.contains(fx0, fy0, fz0) && b.contains(fx1, fy1, fz1) &&
-                                              b.contains(fx0, fy0, fz1) && b.contains(fx0, fy1, fz0) &&
-                                              b.contains(fx0, fy1, fz1) && b.contains(fx1, fy0, fz0) &&
-                                              b.contains(fx1, fy0, fz1) && b.contains(fx1, fy1, fz0);
+        for (; next < 8; next++) {
+            if (!((f.ok >> next) & 1u)) continue;
+            const uint32_t child = f.out[next];
+            if (child == mat) continue;
+            if ((f.outside >> next) & 1u) continue; // set only at height >= 2
+            const bool all = (f.inside >> next) & 1u;
             if (f.height >= 2 && !all) {
-                if (enter(R, b, st[sp + 1], child, f.height - 1, x0, y0, z0)) {
+                const uint32_t cx = next & 1u, cy = (next >> 1) & 1u, cz = next >> 2;
+                const int32_t x0 = int32_t(uint32_t(f.lo[0]) + side * cx), y0 = int32_t(uint32_t(f.lo[1]) + side * cy),
+                              z0 = int32_t(uint32_t(f.lo[2]) + side * cz);
+                if (enter(R, b, st[sp + 1], child, f.height - 1, x0, y0, z0, lane)) {
+                    if (lane == 0) {
+                        f.next = next;
+                        f.changed = changed;
+                    }
                     sp++;
                     pushed = true;
                     break;
                 }
             } else if (all) {
-                f.out.c[id] = mat; // child != mat here
-                f.changed = 1;
+                if (lane == 0) f.out[next] = mat; // child != mat here
+                changed = 1;
             }
-            f.next++;
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (pushed) continue;
-        const uint32_t res = f.changed ? R.intern(f.out) : f.node;
+        uint32_t res = f.node;
+        if (changed) {
+            DNode n;
+#pragma unroll
+            for (int c = 0; c < 8; c++) n.c[c] = f.out[c];
+            res = R.intern(n);
+        }
         if (sp == 0 || res == kOverflow) return res;
         sp--;
         ret = res;
@@ -370,11 +402,15 @@ __device__ uint32_t node_at(const uint32_t* __restrict__ nodes, uint32_t root, i
 
 __global__ void __launch_bounds__(64) k_edit_regions(RegionArgs A)
 {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < A.n_run; t += gridDim.x * blockDim.x) {
+    __shared__ Frame st[kMaxRegionHeight + 1];
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t t = blockIdx.x; t < A.n_run; t += gridDim.x) { // one region per wave
         const uint32_t r = A.run[t];
         Region R{ A, A.pool_off[r], A.cap[r] };
         uint32_t* const tab = A.table + R.poff * 2;
-        for (uint64_t i = 0; i < uint64_t(R.cap) * 2; i++) tab[i] = kEmptySlot;
+        for (uint64_t i = lane; i < uint64_t(R.cap) * 2; i += 64) tab[i] = kEmptySlot;
+        __threadfence_block();
+        __syncthreads();
         const DRegionKey k = A.keys[r];
         uint32_t node = node_at(A.nodes, A.root, A.hs, k);
         const int32_t lx = int32_t((k.x << A.hs) - 0x80000000u), ly = int32_t((k.y << A.hs) - 0x80000000u),
@@ -383,15 +419,18 @@ __global__ void __launch_bounds__(64) k_edit_regions(RegionArgs A)
         for (uint64_t j = A.first[r]; j < A.first[r + 1]; j++) {
             const cqx_sphere_edit e = A.edits[A.edit_ids[j]];
             const Brush b(e);
-            node = fill(R, b, e.material, node, A.hs, lx, ly, lz);
+            node = fill(R, b, e.material, node, uint32_t(A.hs), lx, ly, lz, st, lane);
             if (node == kOverflow) {
                 over = 1;
                 break;
             }
         }
-        A.results[r] = node;
-        A.counts[r] = R.count;
-        A.overflow[r] = over;
+        if (lane == 0) {
+            A.results[r] = node;
+            A.counts[r] = R.count;
+            A.overflow[r] = over;
+        }
+        __syncthreads();
     }
 }
 
@@ -878,6 +917,7 @@ extern "C" {
 cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sphere_edit* edits, uint64_t n,
                                    uint64_t* first_new)
 {
+    const cqx::TraceRange range("cqx_volume_edit_spheres");
     cqx_status st = check_args(ctx, vol);
     if (st != CQX_OK) return st;
     if (n && !edits) return fail(CQX_ERR_INVALID_ARG, "null edits");
@@ -891,6 +931,13 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     if (hs < 0)
         return fail(CQX_ERR_LIMIT, "on-device edits take batches of at least 16 spheres, each within 2^23 voxels of the "
                                    "origin and with radius < 2^20 / 2; apply this batch with cqx_hv_edit_spheres");
+    using clock = std::chrono::steady_clock;
+    const auto ms_since = [](clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(clock::now() - t).count();
+    };
+    clock::time_point t_phase = clock::now();
+    double ms[6] = { 0, 0, 0, 0, 0, 0 }; // plan, index, regions, merge, levels above, SubDAGs
+    int attempts = 0;
     cqx::EditPlan plan;
     try {
         cqx::plan_edits(edits, n, hs, plan);
@@ -901,8 +948,13 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     HIP_TRY(hipDeviceSynchronize()); // launches in flight may read the store or the index
     const uint64_t base = vol->node_count;
     const uint32_t nr = uint32_t(plan.keys.size());
+    ms[0] = ms_since(t_phase);
+    t_phase = clock::now();
     st = ensure_index(ctx, vol, base);
     if (st != CQX_OK) return st;
+    HIP_TRY(hipDeviceSynchronize());
+    ms[1] = ms_since(t_phase);
+    t_phase = clock::now();
 
     // ---- regions ----
     std::vector<DRegionKey> keys(nr);
@@ -910,8 +962,22 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     std::vector<uint32_t> cap(nr), run(nr);
     std::vector<uint64_t> pool_off(nr);
     uint64_t pool_total = 0;
+    // First slice of each region's list: per sphere about twice the voxels of its largest cross
+    // section the region can hold (a sphere changes nodes along its surface, a few per voxel
+    // column), so that few regions have to run again.
+    // CQX_EDIT_FIRST_SLICE=<nodes> fixes the first slices instead (the tests use it to make
+    // regions outgrow them).
+    const char* fixed_slice = std::getenv("CQX_EDIT_FIRST_SLICE");
+    const double fixed = fixed_slice ? std::atof(fixed_slice) : 0.0;
+    const double side = std::ldexp(1.0, hs);
     for (uint32_t r = 0; r < nr; r++) {
-        cap[r] = uint32_t(pow2_at_least(std::max<uint64_t>(64, 128 * (plan.first[r + 1] - plan.first[r]))));
+        double est = 64.0;
+        for (uint64_t j = plan.first[r]; j < plan.first[r + 1]; j++) {
+            const double rad = edits[plan.edit_ids[j]].radius;
+            est += 2.0 * std::min(3.14159265 * rad * rad, side * side) + 16.0;
+        }
+        if (fixed >= 1.0) est = fixed;
+        cap[r] = uint32_t(pow2_at_least(uint64_t(std::min(est, 1048576.0))));
         pool_off[r] = pool_total;
         pool_total += cap[r];
         run[r] = r;
@@ -957,7 +1023,8 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
         RegionArgs A{ vol->nodes, E->index, E->index_cap - 1, vol->root, hs, uint32_t(base), d_edits, d_keys, d_first,
                       d_edit_ids, d_run, uint32_t(run.size()), d_pool_off, d_cap, d_pool, d_table, d_counts, d_results,
                       d_overflow };
-        hipLaunchKernelGGL(k_edit_regions, dim3(grid_for(run.size(), ctx, 64)), dim3(64), 0, 0, A);
+        const uint32_t waves = uint32_t(std::min<uint64_t>(run.size(), uint64_t(ctx->cu_count) * 32));
+        hipLaunchKernelGGL(k_edit_regions, dim3(waves), dim3(64), 0, 0, A);
         HIP_TRY(hipGetLastError());
         std::vector<uint32_t> over(nr);
         HIP_TRY(hipMemcpy(over.data(), d_overflow, nr * 4, hipMemcpyDeviceToHost));
@@ -973,7 +1040,10 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
             cqx::log_debug("device edits: %zu of %zu regions outgrew their node lists; running them again",
                            again.size(), run.size());
         run.swap(again);
+        attempts = attempt + 1;
     }
+    ms[2] = ms_since(t_phase);
+    t_phase = clock::now();
 
     // ---- merge ----
     uint64_t *d_off, *d_slot_of, *d_fin, *d_scalars;
@@ -1047,6 +1117,10 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     }
     E->indexed = base + kept_total;
 
+    HIP_TRY(hipDeviceSynchronize());
+    ms[3] = ms_since(t_phase);
+    t_phase = clock::now();
+
     // ---- the levels above the regions ----
     uint32_t *d_lvl_off, *d_cbeg, *d_cend, *d_morton, *d_res, *d_cand, *d_ckey, *d_ttable, *d_root;
     DRegionKey* d_tkeys;
@@ -1078,8 +1152,16 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     }
     vol->node_count = out[0];
     E->indexed = out[0];
+    ms[4] = ms_since(t_phase);
+    t_phase = clock::now();
     st = set_root(ctx, vol, out[1]);
     if (st != CQX_OK) return st;
+    ms[5] = ms_since(t_phase);
+    cqx::log_debug("device edits: %llu spheres over %u regions of side 2^%d (%d runs); %llu listed, %llu appended, "
+                   "%llu above; plan %.1f ms, index %.1f ms, regions %.1f ms, merge %.1f ms, levels above %.1f ms, "
+                   "SubDAGs %.1f ms", (unsigned long long)n, nr, hs, attempts, (unsigned long long)total,
+                   (unsigned long long)kept_total, (unsigned long long)(out[0] - base - kept_total), ms[0], ms[1],
+                   ms[2], ms[3], ms[4], ms[5]);
     E->roots.resize(E->current + 1);
     E->roots[E->current] = vol->root;
     return CQX_OK;
