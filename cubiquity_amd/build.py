@@ -76,7 +76,8 @@ def build(verbose=False):
     if errors:
         raise RuntimeError("compile failed:\n" + "\n\n".join(errors))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-pthread"], verbose)
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs +
+         ["-pthread", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     build_cpp_checks(verbose)
     return LIB

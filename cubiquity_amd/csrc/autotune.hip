@@ -91,6 +91,7 @@ extern "C" {
 cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
                         float max_footprint, cqx_hit* hits, void* stream, cqx_launch_cfg* best_out, float* best_ms)
 {
+    const cqx::TraceRange range("cqx_autotune");
     if (!ctx || !vol || !rays || !hits) return fail(CQX_ERR_INVALID_ARG, "null argument");
     if (n == 0) return fail(CQX_ERR_INVALID_ARG, "no rays to tune on");
     if (flags & ~CQX_SURFACE_PROPERTIES) return fail(CQX_ERR_INVALID_ARG, "unknown flags");

@@ -464,6 +464,7 @@ extern "C" {
 cqx_status cqx_demo_trace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                           const cqx_demo_params* params, const cqx_launch_cfg* cfg, float* out_rgb, void* stream)
 {
+    const cqx::TraceRange range("cqx_demo_trace");
     if (!ctx || !vol || !params) return fail(CQX_ERR_INVALID_ARG, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     if (n == 0) return CQX_OK;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "cqx_internal.h"
+#include "trace_ranges.h"
 #include "traverse_core.h"
 
 #include <hip/hip_runtime.h>

@@ -675,6 +675,7 @@ extern "C" {
 cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* pp, uint32_t mode,
                          const cqx_launch_cfg* cfg, float* image, void* stream_ptr)
 {
+    const cqx::TraceRange range("cqx_pathtrace");
     cqx_status st = check(ctx, vol, pp);
     if (st != CQX_OK) return st;
     if (!image) return fail(CQX_ERR_INVALID_ARG, "null image");
@@ -692,8 +693,10 @@ cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_para
     hipStream_t s = hipStream_t(stream_ptr);
     HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
     if (mode == 0) {
+        const cqx::TraceRange bounces("wavefront graph: camera rays, then per bounce traversal and shading");
         HIP_TRY(hipGraphLaunch(p->exec, s));
     } else {
+        const cqx::TraceRange mega("megakernel: one lane per path");
         const uint32_t block = 256;
         const size_t lds = size_t(vol->max_height + 1) * block * sizeof(uint32_t);
         if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS", lds);
@@ -718,6 +721,7 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
                                  const cqx_launch_cfg* cfg, const float* d_image, float* d_albedo, float* d_emission,
                                  void* stream_ptr)
 {
+    const cqx::TraceRange range("cqx_pathtrace_adjoint");
     cqx_status st = check(ctx, vol, pp);
     if (st != CQX_OK) return st;
     if (!d_image || !d_albedo || !d_emission) return fail(CQX_ERR_INVALID_ARG, "null buffer");
@@ -747,11 +751,17 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     hipStream_t s = hipStream_t(stream_ptr);
     HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
     // replay the paths (the wavefront graph records each path's vertices), then sweep them backwards
-    HIP_TRY(hipGraphLaunch(p->exec, s));
-    hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat, d_image,
-                       pp->n_materials, p->partials);
-    hipLaunchKernelGGL(k_pt_reduce, dim3(pp->n_materials * 6), dim3(256), 0, s, p->partials, blocks,
-                       pp->n_materials, d_albedo, d_emission);
+    {
+        const cqx::TraceRange replay("adjoint: path replay (wavefront graph)");
+        HIP_TRY(hipGraphLaunch(p->exec, s));
+    }
+    {
+        const cqx::TraceRange sweep("adjoint: backward sweep and reduction");
+        hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat, d_image,
+                           pp->n_materials, p->partials);
+        hipLaunchKernelGGL(k_pt_reduce, dim3(pp->n_materials * 6), dim3(256), 0, s, p->partials, blocks,
+                           pp->n_materials, d_albedo, d_emission);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s));
     return CQX_OK;

@@ -867,6 +867,7 @@ cqx_status cqx_volume_info(const cqx_dvolume* vol, const uint32_t** dev_nodes, u
 cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
                          float max_footprint, cqx_hit* hits, const cqx_launch_cfg* cfg, void* stream)
 {
+    const cqx::TraceRange range("cqx_intersect");
     cqx_status st = check_ctx(ctx);
     if (st != CQX_OK) return st;
     if (!vol) return fail(CQX_ERR_INVALID_ARG, "null volume");
@@ -889,6 +890,7 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
 cqx_status cqx_intersect_noalloc(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n, uint32_t flags,
                                  float max_footprint, cqx_hit* hits, const cqx_launch_cfg* cfg, void* stream)
 {
+    const cqx::TraceRange range("cqx_intersect_noalloc");
     // Allocates nothing: validation, the tuned-launch lookup and kernel launches only; the
     // counters and binning buffers are the device's static scratch (cleared by the launches).
     cqx_status st = check_ctx(ctx);
@@ -920,6 +922,7 @@ cqx_status cqx_intersect_noalloc(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
 cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_t n_batches, const cqx_launch_cfg* cfg,
                                  void* stream_ptr)
 {
+    const cqx::TraceRange range("cqx_intersect_batched");
     cqx_status st = check_ctx(ctx);
     if (st != CQX_OK) return st;
     if (n_batches == 0) return CQX_OK;

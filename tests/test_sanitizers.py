@@ -47,7 +47,7 @@ def asan_libs(tmp_path_factory):
                     zip(srcs, objs)))
     lib = os.path.join(d, "libcqx.so")
     _run(["g++", "-shared", "-fsanitize=address,undefined", "-o", lib] + objs + HIP_OBJS +
-         ["-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
+         ["-L/opt/rocm/lib", "-lamdhip64", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib", "-pthread"])
     orc = os.path.join(d, "liboracle.so")
     _run(["gcc", "-std=c11", "-fPIC", "-shared"] + ASAN + COMMON + ["-o", orc] +
          sorted(glob.glob(os.path.join(ROOT, "oracle", "cpu_*.c"))) + ["-lm"])
