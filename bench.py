@@ -251,7 +251,9 @@ def bench_batched(cq, W, ctx, reps):
 def bench_edit_frames(cq, W, ctx, frames):
     """Edit frames on the 2048^3 volume: per frame 1000 sphere edits on the host
     (cqx_hv_edit_spheres, copy-on-write + hash-consing), the append-only device update
-    (cqx_volume_update) and 2^24 incoherent rays traced against the edited volume."""
+    (cqx_volume_update) and 2^24 incoherent rays traced against the edited volume; beside them the
+    same edits applied on the device to a second device copy (cqx_volume_edit_spheres), whose
+    store must equal the host's node for node after every frame."""
     import torch
 
     w = W.EDIT_2048
@@ -261,13 +263,14 @@ def bench_edit_frames(cq, W, ctx, frames):
     rays = torch.from_numpy(cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, w.incoherent_rays)).cuda()
     hits = torch.empty((rays.shape[0], 32), dtype=torch.uint8, device="cuda")
     dv = ctx.upload(hv)
+    dg = ctx.upload(hv)  # edited on the device
     nodes0 = hv.node_count
     trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
     # node entries per ray of the last launch (record word 6), summed on the GPU
     entries = lambda: int(hits.view(torch.int32)[:, 6].to(torch.int64).sum().item()) / rays.shape[0]  # noqa: E731
     before = timed(trace, 3)
     entries_before = entries()
-    edit_ms, update_ms, trace_ms, appended = [], [], [], []
+    edit_ms, update_ms, trace_ms, appended, dev_ms, same = [], [], [], [], [], True
     for f in range(frames):
         e = W.edit_frame_spheres(f)
         t = time.perf_counter()
@@ -278,6 +281,13 @@ def bench_edit_frames(cq, W, ctx, frames):
         update_ms.append((time.perf_counter() - t) * 1e3)
         appended.append(hv.node_count - first)
         trace_ms.append(timed(trace, 3))
+        t = time.perf_counter()
+        gfirst = dg.edit_spheres(e)
+        dev_ms.append((time.perf_counter() - t) * 1e3)
+        same &= (gfirst == first and dg.node_count == hv.node_count and dg.root == hv.root and
+                 np.array_equal(dg.download(first), hv.nodes[first:]))
+        log(f"edit frames: frame {f}: host {edit_ms[-1]:.1f} ms, device {dev_ms[-1]:.1f} ms, stores equal {same}")
+    dg.free()
     entries_after = entries()
     # the same edited volume compacted (bake: breadth-first order, no appended ranges), uploaded again
     nodes_edited = hv.node_count
@@ -296,7 +306,11 @@ def bench_edit_frames(cq, W, ctx, frames):
             "node_entries_per_ray_before_edits": round(entries_before, 3),
             "node_entries_per_ray_after_edits": round(entries_after, 3),
             "trace_ms_after_compaction": round(compacted, 3),
-            "frame_ms": round(float(np.mean(edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 3)}
+            "frame_ms": round(float(np.mean(edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 3),
+            "device_edit_ms_per_frame": [round(x, 2) for x in dev_ms],
+            "device_edit_ms_steady": mean(dev_ms[1:] or dev_ms),
+            "device_store_equals_host_store": bool(same),
+            "frame_ms_device_edits": round(float(np.mean(dev_ms[1:] or dev_ms) + np.mean(trace_ms)), 3)}
 
 
 def bench_primary(cq, W, ctx, reps):
@@ -350,6 +364,7 @@ def bench_large(cq, W, ctx, frames, w=None):
     t0 = time.time()
     dv = ctx.upload(hv)
     upload_s = time.time() - t0
+    dg = ctx.upload(hv)  # a second device copy, edited on the device
     log(f"large: uploaded in {upload_s:.1f}s")
     best, _ = cq.autotune(ctx, dv, rays, True)
     trace = lambda: cq.intersectVolume(ctx, dv, rays, True, out=hits)  # noqa: E731
@@ -361,7 +376,7 @@ def bench_large(cq, W, ctx, frames, w=None):
     log(f"large: traversal {before:.2f} ms, {entries0:.2f} node entries per ray, hit fraction {hitfrac0:.4f}")
     sample_before = oracle_sample()
     nodes0 = hv.node_count
-    edit_ms, update_ms, trace_ms, appended = [], [], [], []
+    edit_ms, update_ms, trace_ms, appended, dev_ms, same = [], [], [], [], [], True
     for f in range(frames):
         e = W.edit_frame_spheres(f, log2_size=w.log2_size, n_materials=255, n=W.LARGE_FRAME_EDITS)
         t = time.perf_counter()
@@ -373,10 +388,17 @@ def bench_large(cq, W, ctx, frames, w=None):
         appended.append(hv.node_count - first)
         trace_ms.append(timed(trace, 2))
         entries1, hitfrac1 = per_ray()
-        log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms, update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms, "
+        t = time.perf_counter()
+        gfirst = dg.edit_spheres(e)
+        dev_ms.append((time.perf_counter() - t) * 1e3)
+        same &= (gfirst == first and dg.node_count == hv.node_count and dg.root == hv.root and
+                 np.array_equal(dg.download(first), hv.nodes[first:]))
+        log(f"large: frame {f}: edits {edit_ms[-1]:.0f} ms (device {dev_ms[-1]:.1f} ms, stores equal {same}), "
+            f"update {update_ms[-1]:.0f} ms, trace {trace_ms[-1]:.1f} ms, "
             f"{entries1:.2f} node entries per ray, hit fraction {hitfrac1:.4f}")
     sample_after = oracle_sample()
     dv.free()
+    dg.free()
     mean = lambda v: round(float(np.mean(v)), 3)  # noqa: E731
     n = rays.shape[0]
     return {"volume": f"{w.side}^3 seed {w.seed}, {w.params().n_spheres} spheres, 255 materials", "nodes": nodes0, "store_gb": round(nodes0 * 32 / 1e9, 2),
@@ -391,6 +413,10 @@ def bench_large(cq, W, ctx, frames, w=None):
             "oracle_sample_rays": 1 << 20, "oracle_sample_equal_before_edits": sample_before,
             "oracle_sample_equal_after_edits": sample_after,
             "frame_ms_steady": round(float(np.mean(edit_ms[1:] or edit_ms) + np.mean(update_ms) + np.mean(trace_ms)), 1),
+            "device_edit_ms_per_frame": [round(x, 1) for x in dev_ms],
+            "device_edit_ms_steady": mean(dev_ms[1:] or dev_ms),
+            "device_store_equals_host_store": bool(same),
+            "frame_ms_steady_device_edits": round(float(np.mean(dev_ms[1:] or dev_ms) + np.mean(trace_ms)), 1),
             "launch": {f2: getattr(best, f2) for f2, _ in best._fields_}}
 
 
