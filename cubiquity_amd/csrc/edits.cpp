@@ -405,7 +405,40 @@ void plan_edits(const cqx_sphere_edit* edits, uint64_t n, int hs, EditPlan& plan
             for (uint32_t y = lo[1]; y <= hi[1]; y++)
                 for (uint32_t x = lo[0]; x <= hi[0]; x++) touch.push_back({ RegionKey{ x, y, z }, uint32_t(i) });
     }
-    std::stable_sort(touch.begin(), touch.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    // Stable sort by region key (the edits of a region stay in order): an LSD radix sort on the key
+    // as one integer relative to the smallest key, when it spans less than 2^40; otherwise a
+    // comparison sort.
+    uint32_t kmin[3] = { UINT32_MAX, UINT32_MAX, UINT32_MAX }, kmax[3] = { 0, 0, 0 };
+    for (const auto& t : touch) {
+        const uint32_t k[3] = { t.first.x, t.first.y, t.first.z };
+        for (int a = 0; a < 3; a++) {
+            kmin[a] = std::min(kmin[a], k[a]);
+            kmax[a] = std::max(kmax[a], k[a]);
+        }
+    }
+    uint64_t span[3];
+    for (int a = 0; a < 3; a++) span[a] = touch.empty() ? 1 : uint64_t(kmax[a] - kmin[a]) + 1;
+    if (!touch.empty() && span[0] * span[1] < (uint64_t(1) << 40) / span[2]) {
+        std::vector<uint64_t> key(touch.size());
+        for (size_t j = 0; j < touch.size(); j++)
+            key[j] = (uint64_t(touch[j].first.x - kmin[0]) * span[1] + (touch[j].first.y - kmin[1])) * span[2] +
+                     (touch[j].first.z - kmin[2]);
+        std::vector<uint32_t> idx(touch.size()), tmp(touch.size());
+        for (size_t j = 0; j < idx.size(); j++) idx[j] = uint32_t(j);
+        const uint64_t top = span[0] * span[1] * span[2];
+        for (int shift = 0; shift < 40 && (top >> shift) > 0; shift += 11) {
+            std::vector<uint32_t> count(2049, 0);
+            for (uint32_t j : idx) count[((key[j] >> shift) & 2047u) + 1]++;
+            for (int d = 0; d < 2048; d++) count[d + 1] += count[d];
+            for (uint32_t j : idx) tmp[count[(key[j] >> shift) & 2047u]++] = j;
+            idx.swap(tmp);
+        }
+        std::vector<std::pair<RegionKey, uint32_t>> sorted(touch.size());
+        for (size_t j = 0; j < idx.size(); j++) sorted[j] = touch[idx[j]];
+        touch.swap(sorted);
+    } else {
+        std::stable_sort(touch.begin(), touch.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    }
     plan.edit_ids.resize(touch.size());
     for (size_t j = 0; j < touch.size(); j++) {
         plan.edit_ids[j] = touch[j].second;

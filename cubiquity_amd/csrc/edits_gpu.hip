@@ -196,8 +196,9 @@ struct RegionArgs {
     const DRegionKey* keys;
     const uint64_t* first;    // keys + 1 offsets into edit_ids
     const uint32_t* edit_ids;
-    const uint32_t* run; // the regions this launch runs
+    const uint32_t* run; // the regions this launch runs, heaviest first
     uint32_t n_run;
+    uint32_t* next_run;  // work counter: the next entry of `run` to take (zero at launch)
     const uint64_t* pool_off; // each region's slice: pool nodes [pool_off, pool_off + cap)
     const uint32_t* cap;      // power of two
     uint32_t* pool;           // listed nodes, 8 words each
@@ -404,7 +405,11 @@ __global__ void __launch_bounds__(64) k_edit_regions(RegionArgs A)
 {
     __shared__ Frame st[kMaxRegionHeight + 1];
     const uint32_t lane = threadIdx.x;
-    for (uint32_t t = blockIdx.x; t < A.n_run; t += gridDim.x) { // one region per wave
+    for (;;) { // one region per wave, taken from the work counter: the heaviest start first
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(A.next_run, 1u);
+        t = uint32_t(__shfl(int(t), 0));
+        if (t >= A.n_run) break;
         const uint32_t r = A.run[t];
         Region R{ A, A.pool_off[r], A.cap[r] };
         uint32_t* const tab = A.table + R.poff * 2;
@@ -795,7 +800,7 @@ enum : int {
     kBufEdits, kBufKeys, kBufFirst, kBufEditIds, kBufRun, kBufPoolOff, kBufCap, kBufCounts, kBufResults,
     kBufOverflow, kBufPool, kBufTable, kBufOff, kBufSrc, kBufRegion, kBufLevel, kBufKey, kBufSlotOf, kBufRep,
     kBufMergeTable, kBufFin, kBufScalars, kBufLvlOff, kBufTopKeys, kBufCbeg, kBufCend, kBufMorton, kBufRes,
-    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot
+    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot, kBufNext
 };
 
 // Buffer `slot` of the volume's edit state with at least `bytes` (contents not kept when grown).
@@ -961,6 +966,7 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     for (uint32_t r = 0; r < nr; r++) keys[r] = { plan.keys[r].x, plan.keys[r].y, plan.keys[r].z };
     std::vector<uint32_t> cap(nr), run(nr);
     std::vector<uint64_t> pool_off(nr);
+    std::vector<double> work(nr);
     uint64_t pool_total = 0;
     // First slice of each region's list: per sphere about twice the voxels of its largest cross
     // section the region can hold (a sphere changes nodes along its surface, a few per voxel
@@ -976,12 +982,16 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
             const double rad = edits[plan.edit_ids[j]].radius;
             est += 2.0 * std::min(3.14159265 * rad * rad, side * side) + 16.0;
         }
+        work[r] = est;
         if (fixed >= 1.0) est = fixed;
         cap[r] = uint32_t(pow2_at_least(uint64_t(std::min(est, 1048576.0))));
         pool_off[r] = pool_total;
         pool_total += cap[r];
         run[r] = r;
     }
+    // the wave that takes the heaviest region (most sphere surface) early finishes early: the
+    // phase then lasts about as long as its heaviest region or its average load per wave
+    std::stable_sort(run.begin(), run.end(), [&](uint32_t a, uint32_t b) { return work[a] > work[b]; });
     cqx_sphere_edit* d_edits;
     DRegionKey* d_keys;
     uint64_t *d_first, *d_pool_off;
@@ -1020,9 +1030,12 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
         if ((st = upload(E, kBufRun, run, &d_run)) != CQX_OK || (st = upload(E, kBufPoolOff, pool_off, &d_pool_off)) != CQX_OK ||
             (st = upload(E, kBufCap, cap, &d_cap)) != CQX_OK)
             return st;
+        uint32_t* d_next = nullptr;
+        if ((st = buf(E, kBufNext, 1, &d_next)) != CQX_OK) return st;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, 0, d_next, uint64_t(1), 0u);
         RegionArgs A{ vol->nodes, E->index, E->index_cap - 1, vol->root, hs, uint32_t(base), d_edits, d_keys, d_first,
-                      d_edit_ids, d_run, uint32_t(run.size()), d_pool_off, d_cap, d_pool, d_table, d_counts, d_results,
-                      d_overflow };
+                      d_edit_ids, d_run, uint32_t(run.size()), d_next, d_pool_off, d_cap, d_pool, d_table, d_counts,
+                      d_results, d_overflow };
         const uint32_t waves = uint32_t(std::min<uint64_t>(run.size(), uint64_t(ctx->cu_count) * 32));
         hipLaunchKernelGGL(k_edit_regions, dim3(waves), dim3(64), 0, 0, A);
         HIP_TRY(hipGetLastError());
