@@ -44,7 +44,7 @@ class LaunchCfg(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_uint32), ("block", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32),
                 ("regroup_interval", ctypes.c_uint32), ("refill_lanes", ctypes.c_uint32),
                 ("max_iters", ctypes.c_uint32), ("lds_nodes", ctypes.c_uint32),
-                ("bin_bits", ctypes.c_uint32)]
+                ("bin_bits", ctypes.c_uint32), ("stack_rows", ctypes.c_uint32)]
 
 
 class PtParams(ctypes.Structure):

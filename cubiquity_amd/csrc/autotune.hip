@@ -10,9 +10,11 @@
 // configuration is left at "auto" (cfg == NULL or cfg->variant == 0), also for volumes and batches
 // within 2 of a tuned height and class (tuned_cfg).
 //
-// The traversal stack depth is not a free parameter: it is max SubDAG height + 1 LDS rows, the
-// least that holds every node the walk can pop back to. A shorter stack would need restarts that
-// change the sequence of visited nodes (and the reported node-entry counts).
+// The traversal stack depth is tuned too, for the persistent variants 2 and 3: the full stack
+// (max SubDAG height + 1 LDS rows per lane) or a short one of 4, 8 or 16 rows, reused modulo
+// its depth, whose pops past the kept rows descend again from the SubDAG root. Both visit the
+// same nodes in the same order (restart descents are not node entries), so the choice changes
+// LDS per wave and time only.
 #include "device_internal.h"
 
 #include <algorithm>
@@ -117,6 +119,8 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
             if (cur.*field == vals[i]) continue;
             cqx_launch_cfg c = cur;
             c.*field = vals[i];
+            if (c.stack_rows && c.variant != 2 && c.variant != 3) continue; // short stacks: variants 2, 3
+            if (c.stack_rows > uint32_t(vol->max_height)) continue;          // not shorter than the full stack
             float ms = 0;
             const cqx_status r = time_cfg(ctx, vol, rays, n, flags, max_footprint, hits, c, s, ms);
             if (r == CQX_ERR_LIMIT) continue; // e.g. the LDS stack does not fit this block size
@@ -130,6 +134,7 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
     };
     const uint32_t variants[] = { 2, 3, 5 };
     const uint32_t bins[] = { 0, 2, 3, 4 };
+    const uint32_t stacks[] = { 0, 4, 8, 16 };
     for (int pass = 0; pass < 2; pass++) {
         if ((st = sweep(&cqx_launch_cfg::variant, variants, 3)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::bin_bits, bins, 4)) != CQX_OK) return st;
@@ -137,6 +142,7 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
         if ((st = sweep(&cqx_launch_cfg::regroup_interval, regroups, 6)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::refill_lanes, refills, 5)) != CQX_OK) return st;
         if ((st = sweep(&cqx_launch_cfg::block, blocks, 3)) != CQX_OK) return st;
+        if ((st = sweep(&cqx_launch_cfg::stack_rows, stacks, 4)) != CQX_OK) return st;
     }
     // the one-ray-per-lane kernel as the alternative
     for (uint32_t b : blocks) {

@@ -144,12 +144,14 @@ struct BatchSource {
 __device__ __forceinline__ const TraceParams& uniform_params(const SingleSource& s, const TraceParams&) { return s.P; }
 __device__ __forceinline__ const TraceParams& uniform_params(const BatchSource&, const TraceParams& lane) { return lane; }
 
-template <typename Source, bool kCache, bool kTop = false>
+template <typename Source, bool kCache, bool kTop = false, bool kShort = false>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
                                                               uint32_t regroup, uint32_t refill, uint32_t rows,
                                                               uint32_t top_nodes, uint32_t chunk)
 {
+    // kShort: `rows` (a power of two) is the short stack's depth; rows are reused modulo it
+    const uint32_t smask = kShort ? rows - 1u : 0u;
     extern __shared__ uint32_t lds_stack[];
     // kTop: after the stacks, a copy of internal nodes [256, 256 + top_nodes) (the top levels of
     // a breadth-first store, read by most rays) so that those child reads stay out of L2.
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
         const uint64_t walking = __ballot(S.phase == kWalk);
         const uint32_t nwait = 64u - uint32_t(__popcll(walking));
         if (nwait >= refill || walking == 0) {
-            enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
+            enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask);
             if (is_finished(S.phase)) {
                 cqx_hit out;
                 emit_record(uniform_params(src, P), S, out);
@@ -219,7 +221,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                     const float* r;
                     src.fetch(k, P, r, dst);
                     setup_from(S, r);
-                    enter_octants<kCache>(uniform_params(src, P), S, stack, stride);
+                    enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask);
                 }
             }
             const uint64_t nidle = uint64_t(__popcll(idle));
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
         }
         const TraceParams& PP = uniform_params(src, P);
         for (uint32_t s = 0; s < regroup; s++)
-            if (S.phase == kWalk) walk_step<kCache, kTop>(PP, S, stack, stride, top, top_nodes);
+            if (S.phase == kWalk) walk_step<kCache, kTop, kShort>(PP, S, stack, stride, top, top_nodes, smask);
     }
 }
 
@@ -607,11 +609,22 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
     const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
     if (block % 64 != 0 || block == 0 || block > 256)
         return fail(CQX_ERR_INVALID_ARG, "block %u must be a multiple of 64, <= 256", block);
-    const uint32_t rows = uint32_t(vol->max_height) + 1;
+    const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
+    uint32_t rows = uint32_t(vol->max_height) + 1;
+    bool short_stack = false;
+    if (cfg && cfg->stack_rows) {
+        const uint32_t k = cfg->stack_rows;
+        if (k < 2 || (k & (k - 1)) != 0) return fail(CQX_ERR_INVALID_ARG, "stack_rows %u must be a power of two >= 2", k);
+        if (variant != 2 && variant != 3)
+            return fail(CQX_ERR_INVALID_ARG, "short stacks (stack_rows) need variant 2 or 3");
+        if (k < rows) {
+            rows = k;
+            short_stack = true;
+        }
+    }
     const size_t lds = size_t(rows) * block * sizeof(uint32_t);
     if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
 
-    const uint32_t variant = (cfg && cfg->variant) ? cfg->variant : 2u;
     const uint32_t bin_bits = cfg ? cfg->bin_bits : 0u;
     const uint32_t* order = nullptr;
     if (bin_bits) {
@@ -655,7 +668,13 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         }
         SingleSource src{ P, rays, hits, order };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
-        if (variant == 2)
+        if (variant == 2 && short_stack)
+            hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false, false, true>), dim3(blocks), dim3(block), lds,
+                               stream, src, n, n_dev, counters, regroup, refill, rows, 0u, chunk);
+        else if (variant == 3 && short_stack)
+            hipLaunchKernelGGL((k_intersect_persistent<SingleSource, true, false, true>), dim3(blocks), dim3(block), lds,
+                               stream, src, n, n_dev, counters, regroup, refill, rows, 0u, chunk);
+        else if (variant == 2)
             hipLaunchKernelGGL((k_intersect_persistent<SingleSource, false>), dim3(blocks), dim3(block), lds, stream, src,
                                n, n_dev, counters, regroup, refill, rows, 0u, chunk);
         else if (variant == 3)
@@ -950,7 +969,18 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
     if (total > (uint64_t(1) << 38)) return fail(CQX_ERR_LIMIT, "more than 2^38 rays in one launch");
     const uint32_t block = (cfg && cfg->block) ? cfg->block : 256u;
     if (block % 64 != 0 || block == 0 || block > 256) return fail(CQX_ERR_INVALID_ARG, "bad block size %u", block);
-    const uint32_t rows = uint32_t(max_height) + 1;
+    uint32_t rows = uint32_t(max_height) + 1;
+    bool short_stack = false;
+    if (cfg && cfg->stack_rows) {
+        const uint32_t k = cfg->stack_rows;
+        if (k < 2 || (k & (k - 1)) != 0) return fail(CQX_ERR_INVALID_ARG, "stack_rows %u must be a power of two >= 2", k);
+        if (cfg->variant != 0 && cfg->variant != 2 && cfg->variant != 3)
+            return fail(CQX_ERR_INVALID_ARG, "short stacks (stack_rows) need variant 2 or 3");
+        if (k < rows) {
+            rows = k;
+            short_stack = true;
+        }
+    }
     const size_t lds = size_t(rows) * block * sizeof(uint32_t);
     if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
     hipStream_t s = hipStream_t(stream_ptr);
@@ -972,7 +1002,14 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
                      (cfg && cfg->max_iters) ? cfg->max_iters : cqx::kDefaultMaxIters };
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned long long*>(ctx->d_counter));
     unsigned long long* counters = reinterpret_cast<unsigned long long*>(ctx->d_counter);
-    if (cfg && cfg->variant == 3)
+    const bool cache = cfg && cfg->variant == 3;
+    if (cache && short_stack)
+        hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true, false, true>), dim3(blocks), dim3(block), lds, s,
+                           src, total, (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u, chunk);
+    else if (short_stack)
+        hipLaunchKernelGGL((k_intersect_persistent<BatchSource, false, false, true>), dim3(blocks), dim3(block), lds, s,
+                           src, total, (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u, chunk);
+    else if (cache)
         hipLaunchKernelGGL((k_intersect_persistent<BatchSource, true>), dim3(blocks), dim3(block), lds, s, src, total,
                            (const uint32_t*)nullptr, counters, regroup, refill, rows, 0u, chunk);
     else

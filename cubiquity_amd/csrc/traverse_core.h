@@ -117,6 +117,10 @@ struct RayState {
     uint32_t entries, iters;
     uint32_t phase;
     uint32_t nc[8]; // the current node's 8 children (node-cache variant only)
+    // Short stack only: the SubDAG's root node, and the highest height whose row is valid
+    // (rows of heights height .. stack_top hold the walk's ancestors).
+    uint32_t root;
+    int32_t stack_top;
 };
 
 // Node cache: the whole 32-byte node is loaded when the walk enters it (SubDAG root, descent,
@@ -187,10 +191,23 @@ __device__ __forceinline__ void octant_advance(RayState& S)
     S.phase = step == 0 ? kAnomaly : (S.oct > 7 ? kMiss : kOctant);
 }
 
+// Short stacks: `smask` + 1 rows per lane (a power of two, fewer than the SubDAG's height + 1), the
+// ancestor at height h in row h & smask. A push at height h takes the row of height h + smask + 1;
+// a pop to a height whose row was taken descends again from the SubDAG root along the current
+// position (the ancestor at a height is the one node on that path), filling the rows of the
+// heights it passes. The restart's reads are not node entries or iterations of the walk: the
+// records are those of the full stack.
+template <bool kShort>
+__device__ __forceinline__ uint32_t stack_row(int32_t h, uint32_t smask)
+{
+    return kShort ? uint32_t(h) & smask : uint32_t(h);
+}
+
 // Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
 // (:220-250), or move on to the next octant.
-template <bool kCache = false>
-__device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+template <bool kCache = false, bool kShort = false>
+__device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                             uint32_t smask = 0)
 {
     const cqx_subdag sd = P.sub[S.oct ^ S.sign_bits];
     if (sd.node > 0) {
@@ -207,7 +224,13 @@ __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, 
         if (entry < exitt) {
             S.start = sd.height;
             S.height = sd.height;
-            for (int h = 0; h <= S.start; h++) stack[h * stride] = 0;
+            if (kShort) {
+                S.root = sd.node;
+                S.stack_top = sd.height;
+                stack[stack_row<true>(sd.height, smask) * stride] = 0;
+            } else {
+                for (int h = 0; h <= S.start; h++) stack[h * stride] = 0;
+            }
             S.node = sd.node;
             if (kCache) load_node(P, S.node, S.nc);
             S.csize = int32_t(node_size / 2);
@@ -243,9 +266,9 @@ __device__ __forceinline__ uint32_t walk_fetch(const TraceParams& P, const RaySt
 }
 
 // One iteration of the ESVO loop, raytracing.cpp:253-367, given the child that walk_fetch read.
-template <bool kCache = false>
+template <bool kCache = false, bool kShort = false>
 __device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                             uint32_t child)
+                                             uint32_t child, uint32_t smask = 0)
 {
     if (S.iters >= P.max_iters) {
         S.phase = kAnomaly;
@@ -267,7 +290,10 @@ __device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, 
             const float t0x = (float(S.cpx) - S.ox) * S.ix, t0y = (float(S.cpy) - S.oy) * S.iy,
                         t0z = (float(S.cpz) - S.oz) * S.iz;
             const float tce = max3f(t0x, t0y, t0z);
-            if (tcx < S.last_exit) stack[S.height * stride] = S.node;
+            if (tcx < S.last_exit) {
+                stack[stack_row<kShort>(S.height, smask) * stride] = S.node;
+                if (kShort && S.height + int32_t(smask) < S.stack_top) S.stack_top = S.height + int32_t(smask);
+            }
             S.last_exit = tcx;
             S.height--;
             S.node = child;
@@ -306,7 +332,21 @@ __device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, 
             S.phase = kLeave;
             return;
         }
-        S.node = stack[height * stride];
+        if (!kShort || height <= S.stack_top) {
+            S.node = stack[stack_row<kShort>(height, smask) * stride];
+        } else {
+            // the row was taken: descend from the SubDAG root to `height` along the current
+            // position (bit h - 1 of the position picks the child at height h)
+            uint32_t node = S.root;
+            for (int32_t h = S.start; h > height; h--) {
+                stack[stack_row<true>(h, smask) * stride] = node;
+                const uint32_t id = bit_at(S.cpx, h - 1) | (bit_at(S.cpy, h - 1) << 1) | (bit_at(S.cpz, h - 1) << 2);
+                node = P.nodes[uint64_t(node) * 8 + (id ^ S.sign_bits)];
+            }
+            stack[stack_row<true>(height, smask) * stride] = node;
+            S.node = node;
+            S.stack_top = height + int32_t(smask) < S.start ? height + int32_t(smask) : S.start;
+        }
         if (kCache) load_node(P, S.node, S.nc);
         const int32_t ns = int32_t(1u << msb);
         S.csize = ns;
@@ -322,20 +362,21 @@ __device__ __forceinline__ void walk_step_on(const TraceParams& P, RayState& S, 
 }
 
 // One iteration of the ESVO loop (read and step).
-template <bool kCache = false, bool kTop = false>
+template <bool kCache = false, bool kTop = false, bool kShort = false>
 __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                          const uint32_t* top = nullptr, uint32_t top_nodes = 0)
+                                          const uint32_t* top = nullptr, uint32_t top_nodes = 0, uint32_t smask = 0)
 {
-    walk_step_on<kCache>(P, S, stack, stride, walk_fetch<kCache, kTop>(P, S, top, top_nodes));
+    walk_step_on<kCache, kShort>(P, S, stack, stride, walk_fetch<kCache, kTop>(P, S, top, top_nodes), smask);
 }
 
 // Moves a lane that is between SubDAGs (phase kLeave or kOctant) into the walk of the next SubDAG
 // the ray enters, or to the end of its walk. At most 8 octants are tried.
-template <bool kCache = false>
-__device__ __forceinline__ void enter_octants(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+template <bool kCache = false, bool kShort = false>
+__device__ __forceinline__ void enter_octants(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                              uint32_t smask = 0)
 {
     if (S.phase == kLeave) octant_advance(S);
-    while (S.phase == kOctant) octant_enter<kCache>(P, S, stack, stride);
+    while (S.phase == kOctant) octant_enter<kCache, kShort>(P, S, stack, stride, smask);
 }
 
 // One traversal step of a lane whose walk is in progress.

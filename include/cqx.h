@@ -326,6 +326,12 @@ typedef struct {
                                   persistent waves take them cell by cell (each XCD a contiguous run
                                   of cells), so that rays in flight together read nearby nodes.
                                   Results are the same, in the caller's order. n < 2^32. */
+    uint32_t stack_rows;       /* variants 2 and 3: 0 = a traversal stack row per node height (the
+                                  volume's SubDAG height + 1, raytracing.cpp:251's stack); a power
+                                  of two K >= 2 below that = a short stack of K rows per lane in LDS,
+                                  reused modulo K, a pop past them descending again from the SubDAG
+                                  root along the ray's position. Same records, less LDS per wave
+                                  (occupancy for deep volumes). */
 } cqx_launch_cfg;
 
 /* intersectVolume over n rays, batched: rays and hits are DEVICE pointers; stream is a

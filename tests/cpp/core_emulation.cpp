@@ -10,6 +10,8 @@
 //               (the LDS node cache of variant 4)
 //   pair        persistent with two rays per lane (variant 5): both slots' child reads are taken
 //               before either slot steps on its read
+//   shortK      persistent with a short stack of K rows per lane (K a power of two): rows reused
+//               modulo K, pops past them restart from the SubDAG root
 //   traceref    traceRayRef per ray (trace_ref_core.h, the cqx_trace_ref kernel's walk); writes
 //               40-byte RayVolumeIntersection records
 // Usage: core_emulation lane|persistent|top|pair|traceref nodes.bin sub.bin rays.bin out.bin surf
@@ -50,7 +52,9 @@ int main(int argc, char** argv)
     }
     const bool use_top = std::string_view(argv[1]) == "top";
     const bool pair = std::string_view(argv[1]) == "pair";
-    const bool persistent = use_top || std::string_view(argv[1]) == "persistent";
+    const bool short_stack = std::string_view(argv[1]).substr(0, 5) == "short";
+    const uint32_t smask = short_stack ? uint32_t(std::atoi(argv[1] + 5)) - 1u : 0u;
+    const bool persistent = use_top || short_stack || std::string_view(argv[1]) == "persistent";
     const std::vector<char> nb = slurp(argv[2]), sb = slurp(argv[3]), rb = slurp(argv[4]);
     if (nb.empty() || sb.size() != 8 * sizeof(cqx_subdag)) return 3;
     TraceParams P;
@@ -144,7 +148,10 @@ int main(int argc, char** argv)
             int walking = 0;
             for (const RayState& s : S) walking += s.phase == kWalk;
             if (64 - walking >= refill || walking == 0) {
-                for (int l = 0; l < 64; l++) enter_octants(P, S[l], &stack[l], 64);
+                for (int l = 0; l < 64; l++) {
+                    if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask);
+                    else enter_octants(P, S[l], &stack[l], 64);
+                }
                 for (int l = 0; l < 64; l++)
                     if (is_finished(S[l].phase)) {
                         emit_record(P, S[l], out[idx[l]]);
@@ -155,7 +162,8 @@ int main(int argc, char** argv)
                         const float* r = rays + 6 * next;
                         idx[l] = next++;
                         ray_setup(S[l], r[0], r[1], r[2], r[3], r[4], r[5]);
-                        enter_octants(P, S[l], &stack[l], 64);
+                        if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask);
+                        else enter_octants(P, S[l], &stack[l], 64);
                     }
                 int live = 0;
                 for (const RayState& s : S) live += s.phase != kIdle;
@@ -165,6 +173,7 @@ int main(int argc, char** argv)
                 for (int l = 0; l < 64; l++)
                     if (S[l].phase == kWalk) {
                         if (use_top) walk_step<false, true>(P, S[l], &stack[l], 64, top.data(), top_nodes);
+                        else if (short_stack) walk_step<false, false, true>(P, S[l], &stack[l], 64, nullptr, 0, smask);
                         else walk_step(P, S[l], &stack[l], 64);
                     }
         }

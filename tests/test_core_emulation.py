@@ -37,7 +37,8 @@ def _run(emu, tmp_path, cq, hv, rays, mode, surf, fp, max_iters, refill=32, regr
 
 
 @pytest.mark.parametrize("mode,refill,regroup", [("lane", 0, 0), ("persistent", 32, 8), ("persistent", 8, 1),
-                                                 ("persistent", 64, 3), ("top", 16, 4), ("pair", 16, 8)])
+                                                 ("persistent", 64, 3), ("top", 16, 4), ("pair", 16, 8),
+                                                 ("short2", 16, 8), ("short4", 32, 4), ("short8", 8, 1)])
 def test_core_matches_oracle_128(emu, tmp_path, cq, oracle, vol128, rays128, mode, refill, regroup):
     sub = oracle.find_subdags(vol128.nodes, vol128.root)
     rays = np.concatenate([rays128["camera"][::5], rays128["incoherent"], rays128["special"]])
@@ -66,15 +67,16 @@ def test_core_small_and_degenerate_volumes(emu, tmp_path, cq, oracle):
         sp[:, :3] += np.asarray(lower, np.float32)
         rays = np.concatenate([_brute_rays(n, lower, 3000, 17), sp]).astype(np.float32)
         sub = oracle.find_subdags(hv.nodes, hv.root)
-        for mode in ("lane", "persistent", "top", "pair"):
+        for mode in ("lane", "persistent", "top", "pair", "short2", "short4"):
             got = _run(emu, tmp_path, cq, hv, rays, mode, True, -1.0, 1 << 16, 16, 4)
             ref = oracle.intersect(hv.nodes, sub, rays, True, -1.0, max_iters=1 << 16)
             assert records_equal(got, ref).all(), f"{name} {mode}"
 
 
-@pytest.mark.parametrize("mode", ["lane", "persistent", "pair"])
+@pytest.mark.parametrize("mode", ["lane", "persistent", "pair", "short4", "short8", "short16"])
 def test_core_deep_sparse_volumes(emu, tmp_path, cq, oracle, mode):
-    """SubDAGs 21-22 levels deep: 23 stack rows per lane."""
+    """SubDAGs 21-22 levels deep: 23 stack rows per lane, or short stacks of 4-16 rows whose pops
+    past them restart from the SubDAG root."""
     from tests.raysets import deep_volumes
 
     for name, (hv, rays) in deep_volumes(cq).items():

@@ -190,6 +190,89 @@ def test_gpu_deep_sparse_volumes(cq, oracle, gpu_ctx, variant):
         dv.free()
 
 
+@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("stack_rows", [2, 4])
+@pytest.mark.parametrize("bin_bits", [0, 3])
+def test_gpu_short_stack_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, rays128, bin_bits, stack_rows,
+                                            variant):
+    """Short stacks of 2 and 4 rows per lane on the 8-level SubDAGs of the 128^3 volume: pops past
+    the kept rows descend again from the SubDAG root; every record (node-entry and iteration counts
+    included) equals the full-stack walk's."""
+    sub = oracle.find_subdags(vol128.nodes, vol128.root)
+    for set_name in ("camera", "incoherent", "special"):
+        rays = rays128[set_name]
+        for surf, fp in ((True, -1.0), (True, 0.0035)):
+            o = oracle.intersect(vol128.nodes, sub, rays, surf, fp, threads=8)
+            cfg = cq.LaunchCfg(variant=variant, stack_rows=stack_rows, bin_bits=bin_bits)
+            g = _gpu(gpu_ctx, dvol128, rays, surf, fp, cfg)
+            eq = records_equal(g, o)
+            assert eq.all(), _report(f"{set_name} fp={fp}", eq, rays, g, o)
+
+
+@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("stack_rows", [2, 4, 8, 16])
+def test_gpu_short_stack_deep_sparse_volumes(cq, oracle, gpu_ctx, variant, stack_rows):
+    """SubDAGs 21-22 levels deep walked with 2-16 stack rows per lane instead of 23."""
+    from tests.raysets import deep_volumes
+
+    for name, (hv, rays) in deep_volumes(cq).items():
+        dv = gpu_ctx.upload(hv)
+        sub = oracle.find_subdags(hv.nodes, hv.root)
+        for surf, fp in ((True, -1.0), (True, 0.01)):
+            o = oracle.intersect(hv.nodes, sub, rays, surf, fp, max_iters=1 << 16)
+            cfg = cq.LaunchCfg(variant=variant, stack_rows=stack_rows, max_iters=1 << 16)
+            g = _gpu(gpu_ctx, dv, rays, surf, fp, cfg)
+            eq = records_equal(g, o)
+            assert eq.all(), _report(f"{name} rows={stack_rows} fp={fp}", eq, rays, g, o)
+        dv.free()
+
+
+def test_gpu_short_stack_small_volumes_and_batched(cq, oracle, gpu_ctx):
+    """Short stacks on the small and degenerate volumes (one launch each), then all of them in one
+    batched launch with a short stack: each batch equals the oracle."""
+    import torch
+
+    from tests.raysets import special_rays
+    from tests.test_oracle_pinning import _brute_rays
+
+    launch, refs = [], []
+    for name, (hv, n, lower) in _small_volumes(cq).items():
+        dv = gpu_ctx.upload(hv)
+        sp = special_rays(n)
+        sp[:, :3] += np.asarray(lower, np.float32)
+        rays = np.concatenate([_brute_rays(n, lower, 3000, 23), sp]).astype(np.float32)
+        o = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, -1.0, max_iters=1 << 16)
+        g = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=2, stack_rows=2, max_iters=1 << 16))
+        eq = records_equal(g, o)
+        assert eq.all(), _report(name, eq, rays, g, o)
+        r = torch.from_numpy(rays).cuda()
+        launch.append((dv, r, torch.zeros((len(rays), 32), dtype=torch.uint8, device="cuda"), True, -1.0))
+        refs.append(o)
+    for variant in (2, 3):
+        for _, _, h, _, _ in launch:
+            h.zero_()
+        cq.intersect_batched(gpu_ctx, launch, cfg=cq.LaunchCfg(variant=variant, stack_rows=4, max_iters=1 << 16))
+        torch.cuda.synchronize()
+        for k, ((_, r, h, _, _), o) in enumerate(zip(launch, refs)):
+            got = h.cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
+            assert records_equal(got, o).all(), f"variant {variant} batch {k}"
+    for dv, _, _, _, _ in launch:
+        dv.free()
+
+
+def test_gpu_short_stack_rejects_bad_configs(cq, gpu_ctx, dvol128, rays128):
+    import torch
+
+    r = torch.from_numpy(rays128["camera"][:256]).cuda()
+    for cfg in (cq.LaunchCfg(variant=2, stack_rows=3), cq.LaunchCfg(variant=2, stack_rows=1),
+                cq.LaunchCfg(variant=1, stack_rows=4), cq.LaunchCfg(variant=5, stack_rows=4)):
+        with pytest.raises(cq.CqxError):
+            cq.intersectVolume(gpu_ctx, dvol128, r, True, cfg=cfg)
+    h = torch.zeros((256, 32), dtype=torch.uint8, device="cuda")
+    with pytest.raises(cq.CqxError):
+        cq.intersect_batched(gpu_ctx, [(dvol128, r, h, True, -1.0)], cfg=cq.LaunchCfg(variant=1, stack_rows=4))
+
+
 def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     """The benchmark workload at full size: all 2^24 incoherent rays bit-exact against the
     oracle; both kernel variants agree; hit voxels hold the hit material (Volume::voxel)."""
@@ -209,7 +292,11 @@ def test_gpu_full_size_1024(cq, oracle, gpu_ctx):
     g5 = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=5))
     g2b = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=2, bin_bits=3))
     g3b = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=3, bin_bits=4))
+    g2s = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=2, stack_rows=4, bin_bits=3))
+    g3s = _gpu(gpu_ctx, dv, rays, True, -1.0, cq.LaunchCfg(variant=3, stack_rows=8))
     print(f"gpu {time.time() - t0:.2f}s", flush=True)
+    assert records_equal(g2s, g2).all()
+    assert records_equal(g3s, g2).all()
     assert records_equal(g2b, g2).all()
     assert records_equal(g3b, g2).all()
     assert records_equal(g1, g2).all()
@@ -241,6 +328,7 @@ def test_gpu_autotuned_launch_is_exact(cq, oracle, gpu_ctx, vol128, rays128):
     r = torch.from_numpy(rays).cuda()
     best, ms = cq.autotune(ctx, dv, r, True, -1.0)
     assert best.variant in (1, 2, 3, 5) and best.bin_bits in (0, 2, 3, 4) and ms > 0
+    assert best.stack_rows in (0, 4, 8) and (best.stack_rows == 0 or best.variant in (2, 3))
     auto = cq.intersectVolume(ctx, dv, r, True).cpu().numpy().view(cq.HIT_DTYPE).reshape(-1)
     o = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True)
     assert records_equal(auto, o).all()
