@@ -528,7 +528,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--rays", type=int, default=0, help="rays of the traversal workload (0: the workload's own count)")
+    ap.add_argument("--workload", default="pathtrace_1024", choices=["pathtrace_1024", "large_16384"],
+                    help="volume and rays of the timed traversal: the 1024^3 headline (default) or the 16384^3, "
+                         "1M-sphere volume with 2^26 rays (host build ~9 min, ~70 GB of host memory; for profiling "
+                         "its traversal)")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pathtrace", action="store_true")
@@ -558,7 +562,12 @@ def main():
     import cubiquity_amd as cq
     from cubiquity_amd import workloads as W
 
-    w = W.PATHTRACE_1024 if args.log2_size == 10 else W.scaled_headline(args.log2_size)
+    if args.workload == "large_16384":
+        w = W.LARGE_16384
+        os.environ["CQX_PROGRESS"] = "1"  # the host build reports its progress
+    else:
+        w = W.PATHTRACE_1024 if args.log2_size == 10 else W.scaled_headline(args.log2_size)
+    args.rays = args.rays or w.incoherent_rays
     t0 = time.time()
     hv = cq.HostVolume.generate(w.params())
     rays = cq.incoherent_rays(hv, w.log2_size, w.incoherent_seed, args.rays)
@@ -621,8 +630,8 @@ def main():
         timed_launch = launch_string(tuned)
     elif args.launch:
         timed_launch = launch_string({k: getattr(cfg, k) for k in LAUNCH_KEYS})
-    traffic, l2_hit_rate, profile = (traffic_from_profiles(timed_launch) if args.log2_size == 10 and
-                                     args.rays == 1 << 24 else (None, None, None))
+    headline = w is W.PATHTRACE_1024 and args.rays == 1 << 24
+    traffic, l2_hit_rate, profile = traffic_from_profiles(timed_launch) if headline else (None, None, None)
 
     value = args.rays * args.steps / wall / 1e6
     out = {
@@ -639,8 +648,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{w.name} volume ({w.side}^3, seed 1) + {args.rays} incoherent rays (seed 7), "
-                        "intersectVolume surface properties on, LOD off",
+            "workload": f"{w.name} volume ({w.side}^3, seed {w.seed}) + {args.rays} incoherent rays "
+                        f"(seed {w.incoherent_seed}), intersectVolume surface properties on, LOD off",
             "rays": args.rays,
             "nodes": hv.node_count,
             "variant": args.variant,
@@ -660,7 +669,7 @@ def main():
             "limiter": ("measured at 1024^3: cache-resident (the 20 MB store sits in the MALL; L2 hits ~91 %, "
                         "HBM traffic ~0.3x the algorithmic bytes), bound by dependent child-read latency and "
                         "divergent VALU work (lane utilisation ~38 %), not by HBM bandwidth; 'hbm' names the "
-                        "roofline the algorithmic bytes are priced against" if args.log2_size == 10 else
+                        "roofline the algorithmic bytes are priced against" if headline else
                         "see DESIGN.md for the measured limiter at this size"),
             "kernel_ms": round(avg_ms, 4),
             "bytes_per_launch": alg_bytes,

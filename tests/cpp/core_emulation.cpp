@@ -14,6 +14,10 @@
 //               modulo K, pops past them restart from the SubDAG root
 //   traceref    traceRayRef per ray (trace_ref_core.h, the cqx_trace_ref kernel's walk); writes
 //               40-byte RayVolumeIntersection records
+//   census      persistent, and also prints a census of the walk loop on stdout: lane slots of
+//               walking and waiting lanes, service passes, and which path each walk step took
+//               (descend, advance, advance + pop, leave the SubDAG, hit or stop), per lane and
+//               per wave step (a wave step runs every path one of its lanes takes)
 // Usage: core_emulation lane|persistent|top|pair|traceref nodes.bin sub.bin rays.bin out.bin surf
 //                       max_footprint max_iters(traceref: root) refill regroup
 #define __device__
@@ -54,7 +58,8 @@ int main(int argc, char** argv)
     const bool pair = std::string_view(argv[1]) == "pair";
     const bool short_stack = std::string_view(argv[1]).substr(0, 5) == "short";
     const uint32_t smask = short_stack ? uint32_t(std::atoi(argv[1] + 5)) - 1u : 0u;
-    const bool persistent = use_top || short_stack || std::string_view(argv[1]) == "persistent";
+    const bool census = std::string_view(argv[1]) == "census";
+    const bool persistent = use_top || short_stack || census || std::string_view(argv[1]) == "persistent";
     const std::vector<char> nb = slurp(argv[2]), sb = slurp(argv[3]), rb = slurp(argv[4]);
     if (nb.empty() || sb.size() != 8 * sizeof(cqx_subdag)) return 3;
     TraceParams P;
@@ -144,10 +149,15 @@ int main(int argc, char** argv)
         uint64_t idx[64] = {};
         for (RayState& s : S) s.phase = kIdle;
         uint64_t next = 0;
+        // census: lane slots, service passes, steps per path (per lane; per wave step with the path)
+        enum { kD, kA, kP, kL, kH, kPaths };
+        uint64_t slots_walk = 0, slots_wait = 0, services = 0, wave_steps = 0, lane_paths[kPaths] = {},
+                 wave_paths[kPaths] = {};
         for (;;) {
             int walking = 0;
             for (const RayState& s : S) walking += s.phase == kWalk;
             if (64 - walking >= refill || walking == 0) {
+                services++;
                 for (int l = 0; l < 64; l++) {
                     if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask);
                     else enter_octants(P, S[l], &stack[l], 64);
@@ -169,14 +179,44 @@ int main(int argc, char** argv)
                 for (const RayState& s : S) live += s.phase != kIdle;
                 if (next >= n && live == 0) break;
             }
-            for (int s = 0; s < regroup; s++)
+            for (int s = 0; s < regroup; s++) {
+                bool any[kPaths] = {};
+                int walked = 0;
                 for (int l = 0; l < 64; l++)
                     if (S[l].phase == kWalk) {
+                        const int32_t h0 = S[l].height;
+                        const uint32_t e0 = S[l].entries;
                         if (use_top) walk_step<false, true>(P, S[l], &stack[l], 64, top.data(), top_nodes);
                         else if (short_stack) walk_step<false, false, true>(P, S[l], &stack[l], 64, nullptr, 0, smask);
                         else walk_step(P, S[l], &stack[l], 64);
+                        const int path = S[l].phase == kLeave ? kL
+                                         : S[l].phase != kWalk ? kH
+                                         : S[l].height < h0    ? kD
+                                         : S[l].entries != e0  ? kP
+                                                               : kA;
+                        lane_paths[path]++;
+                        any[path] = true;
+                        walked++;
                     }
+                slots_walk += uint64_t(walked);
+                slots_wait += uint64_t(64 - walked);
+                if (walked) {
+                    wave_steps++;
+                    for (int k = 0; k < kPaths; k++) wave_paths[k] += any[k] ? 1u : 0u;
+                }
+            }
         }
+        if (census)
+            std::printf("census rays %llu services %llu wave_steps %llu slots_walk %llu slots_wait %llu "
+                        "lane_D %llu lane_A %llu lane_P %llu lane_L %llu lane_H %llu "
+                        "wave_D %llu wave_A %llu wave_P %llu wave_L %llu wave_H %llu\n",
+                        (unsigned long long)n, (unsigned long long)services, (unsigned long long)wave_steps,
+                        (unsigned long long)slots_walk, (unsigned long long)slots_wait,
+                        (unsigned long long)lane_paths[kD], (unsigned long long)lane_paths[kA],
+                        (unsigned long long)lane_paths[kP], (unsigned long long)lane_paths[kL],
+                        (unsigned long long)lane_paths[kH], (unsigned long long)wave_paths[kD],
+                        (unsigned long long)wave_paths[kA], (unsigned long long)wave_paths[kP],
+                        (unsigned long long)wave_paths[kL], (unsigned long long)wave_paths[kH]);
     }
     FILE* f = std::fopen(argv[5], "wb");
     if (!f) return 4;

@@ -104,3 +104,32 @@ def test_trace_ref_core_matches_oracle(emu, tmp_path, cq, oracle, vol128, rays12
         got = np.fromfile(paths[3], dtype=cq.INTERSECTION_DTYPE)
         ref = oracle.trace_ref(hv.nodes, hv.root, rays)
         assert got.tobytes() == ref.tobytes()
+
+
+def test_core_census_of_the_walk_loop(emu, tmp_path, cq, oracle, vol128, rays128):
+    """The census mode (the persistent driver with counters): its records equal the oracle's, every
+    walk step is one iteration of exactly one path, descents + pops + SubDAG entries are the node
+    entries, and the incoherent set keeps every wave step divergent (descend, advance and pop all
+    taken in almost every step) -- the fact the persistent kernel's VALU lane utilisation follows."""
+    sub = cq.findSubDAGs(vol128.nodes, vol128.root)
+    rays = rays128["incoherent"]
+    paths = [str(tmp_path / f) for f in ("n.bin", "s.bin", "r.bin", "o.bin")]
+    np.ascontiguousarray(vol128.nodes).tofile(paths[0])
+    np.ascontiguousarray(sub).tofile(paths[1])
+    np.ascontiguousarray(rays, dtype=np.float32).tofile(paths[2])
+    r = subprocess.run([emu, "census", *paths, "1", "-1.0", str(1 << 16), "16", "16"], check=True, timeout=300,
+                       capture_output=True, text=True)
+    words = r.stdout.split()
+    assert words[0] == "census"
+    c = {k: int(v) for k, v in zip(words[1::2], words[2::2])}
+    got = np.fromfile(paths[3], dtype=cq.HIT_DTYPE)
+    ref = oracle.intersect(vol128.nodes, oracle.find_subdags(vol128.nodes, vol128.root), rays, True, -1.0,
+                           max_iters=1 << 16, threads=8)
+    assert records_equal(got, ref).all()
+    lanes = sum(c[f"lane_{k}"] for k in "DAPLH")
+    assert lanes == c["slots_walk"] == int(ref["iters"].astype(np.int64).sum())
+    subdag_entries = int(ref["entries"].astype(np.int64).sum()) - c["lane_D"] - c["lane_P"]
+    assert 0 < subdag_entries <= 8 * len(rays)
+    assert c["rays"] == len(rays) and c["lane_H"] <= len(rays)
+    for k in "DAP":
+        assert c[f"wave_{k}"] > 0.95 * c["wave_steps"], c
