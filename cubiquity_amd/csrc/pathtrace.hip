@@ -39,8 +39,8 @@ using cqx::release;
 
 namespace {
 
-constexpr int kMaxVertices = 8; // adjoint: bounces + 1 <= 8
-// Path record of the wavefront pass, kept when bounces + 1 <= kMaxVertices: vmat[j * P + p] is
+constexpr int kMaxVertices = 65; // bounces + 1 <= 65 (at most 64 bounces)
+// Path record of the wavefront pass: vmat[j * P + p] is
 // the material of path p's vertex j, vstat[p] its vertex count K, plus kEscaped when the path
 // ended in the sky. Paths that end by an anomaly or a failed scatter have kEscaped clear.
 constexpr uint32_t kEscaped = 0x80u;
@@ -332,6 +332,9 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 
 // The backward sweep over each path's recorded vertices. The throughputs beta_j are rebuilt as the
 // forward pass formed them (beta_0 = 1, beta_{j+1} = beta_j * albedo[m_j], FP32 in that order).
+// The sweep takes the vertices in segments of kSegment, last segment first: a segment's betas are
+// rebuilt from beta_0 by the same products (so a path of up to 64 bounces needs registers for one
+// segment only), then its vertices are swept from last to first.
 //
 // The gradient sums have a fixed order, so the result is the same bit for bit on every run: each
 // wave takes 64 consecutive paths at a time and, vertex slot by vertex slot, groups its lanes by
@@ -340,6 +343,7 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 // sums to the wave's own row of per-material totals in LDS. No atomics are involved; the block
 // adds its waves' rows in wave order.
 constexpr uint32_t kAdjointBlock = 256;
+constexpr int kSegment = 8;
 constexpr uint32_t kAdjointWaves = kAdjointBlock / 64;
 
 __global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams* __restrict__ dp, uint64_t P,
@@ -366,47 +370,61 @@ __global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams*
         const uint32_t pixel = valid ? uint32_t(p) / d.spp : 0u; // P < 2^32
         float delta[3];
         for (int a = 0; a < 3; a++) delta[a] = valid ? dimage[uint64_t(pixel) * 3 + a] * inv_spp : 0.0f;
-        uint32_t vm[kMaxVertices];
-        float vb[kMaxVertices][3];
-        float b[3] = { 1.0f, 1.0f, 1.0f };
-#pragma unroll
-        for (int j = 0; j < kMaxVertices; j++) {
-            vm[j] = 0u;
-            if (j < K) {
-                const uint32_t m = vmat[uint64_t(j) * P + p];
-                vm[j] = m;
-                for (int a = 0; a < 3; a++) {
-                    vb[j][a] = b[a];
-                    b[a] = b[a] * d.albedo[m * 3 + a];
-                }
-            }
-        }
         float G[3];
         for (int a = 0; a < 3; a++) G[a] = escaped ? d.sky[a] : 0.0f;
+        // the wave's most vertices (a wave-uniform segment count)
+        int kw = K;
 #pragma unroll
-        for (int j = kMaxVertices - 1; j >= 0; j--) {
-            const bool live = j < K;
-            const uint32_t m = vm[j];
-            float c[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
-            if (live) {
-                for (int a = 0; a < 3; a++) {
-                    const float w = delta[a] * vb[j][a];
-                    c[a] = w * G[a];
-                    c[3 + a] = w;
-                    G[a] = d.emission[m * 3 + a] + d.albedo[m * 3 + a] * G[a];
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(kw, off);
+            kw = o > kw ? o : kw;
+        }
+        for (int seg = (kw + kSegment - 1) / kSegment - 1; seg >= 0; seg--) {
+            const int j0 = seg * kSegment;
+            float b[3] = { 1.0f, 1.0f, 1.0f };
+            for (int j = 0; j < j0 && j < K; j++) {
+                const uint32_t m = vmat[uint64_t(j) * P + p];
+                for (int a = 0; a < 3; a++) b[a] = b[a] * d.albedo[m * 3 + a];
+            }
+            uint32_t vm[kSegment];
+            float vb[kSegment][3];
+#pragma unroll
+            for (int i = 0; i < kSegment; i++) {
+                vm[i] = 0u;
+                if (j0 + i < K) {
+                    const uint32_t m = vmat[uint64_t(j0 + i) * P + p];
+                    vm[i] = m;
+                    for (int a = 0; a < 3; a++) {
+                        vb[i][a] = b[a];
+                        b[a] = b[a] * d.albedo[m * 3 + a];
+                    }
                 }
             }
-            uint64_t todo = __ballot(live);
-            while (todo) {
-                const uint32_t m0 = uint32_t(__shfl(int(m), __ffsll((unsigned long long)todo) - 1));
-                const bool in = live && m == m0;
-                todo &= ~__ballot(in);
 #pragma unroll
-                for (int k = 0; k < 6; k++) {
-                    float v = in ? c[k] : 0.0f;
+            for (int i = kSegment - 1; i >= 0; i--) {
+                const bool live = j0 + i < K;
+                const uint32_t m = vm[i];
+                float c[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+                if (live) {
+                    for (int a = 0; a < 3; a++) {
+                        const float w = delta[a] * vb[i][a];
+                        c[a] = w * G[a];
+                        c[3 + a] = w;
+                        G[a] = d.emission[m * 3 + a] + d.albedo[m * 3 + a] * G[a];
+                    }
+                }
+                uint64_t todo = __ballot(live);
+                while (todo) {
+                    const uint32_t m0 = uint32_t(__shfl(int(m), __ffsll((unsigned long long)todo) - 1));
+                    const bool in = live && m == m0;
+                    todo &= ~__ballot(in);
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                    if (lane == 0) gw[m0 * 6 + k] += v;
+                    for (int k = 0; k < 6; k++) {
+                        float v = in ? c[k] : 0.0f;
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                        if (lane == 0) gw[m0 * 6 + k] += v;
+                    }
                 }
             }
         }
@@ -460,7 +478,7 @@ struct PtPlan {
     float* L = nullptr;
     uint64_t* rng = nullptr;
     uint32_t* counts = nullptr;
-    uint8_t* vmat = nullptr;  // (bounces + 1) x P vertex materials, when bounces + 1 <= kMaxVertices
+    uint8_t* vmat = nullptr;  // (bounces + 1) x P vertex materials
     uint8_t* vstat = nullptr; // P path records (vertex count, escaped)
     float* partials = nullptr;
     size_t partial_floats = 0;
@@ -614,7 +632,7 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
     if (st == CQX_OK) st = alloc(&p->beta, 3 * P);
     if (st == CQX_OK) st = alloc(&p->rng, P);
     if (st == CQX_OK) st = alloc(&p->counts, 64);
-    if (st == CQX_OK && p->bounces + 1 <= uint32_t(kMaxVertices)) { // the adjoint's path record
+    if (st == CQX_OK) { // the adjoint's path record
         st = alloc(&p->vmat, (p->bounces + 1) * P);
         if (st == CQX_OK) st = alloc(&p->vstat, P);
     }

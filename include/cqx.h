@@ -436,7 +436,7 @@ cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_para
  * paths of cqx_pathtrace are traced again (same RNG streams, wavefront form) with each vertex's
  * material recorded, then swept backwards; geometry and traversal are discrete and exact. d_image = dLoss/dimage (device, height x width
  * x 3); d_albedo, d_emission (device, n_materials x 3) receive dLoss/dalbedo and
- * dLoss/demission. At most 7 bounces. Synchronizes `stream` before returning. */
+ * dLoss/demission. Up to 64 bounces, as cqx_pathtrace. Synchronizes `stream` before returning. */
 cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* params,
                                  const cqx_launch_cfg* cfg, const float* d_image, float* d_albedo,
                                  float* d_emission, void* stream);

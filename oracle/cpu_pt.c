@@ -30,7 +30,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define MAXV 8
+#define MAXV 65 /* vertices of a path: bounces + 1, at most 64 bounces */
 
 typedef struct { uint64_t state, inc; } pcg_t;
 

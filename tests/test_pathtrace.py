@@ -204,11 +204,12 @@ def test_gpu_adjoint_is_bit_reproducible(cq, gpu_ctx, scene16):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bounces", [0, 7])
+@pytest.mark.parametrize("bounces", [0, 7, 8, 17, 64])
 def test_gpu_adjoint_at_bounce_limits(cq, oracle, gpu_ctx, scene16, bounces):
     """The adjoint replays the wavefront pass, which records up to bounces + 1 vertex materials
-    per path, then sweeps the record: at 0 bounces (one vertex) and at 7 (eight, the most the
-    record holds) the gradient must equal the FP64 one over the same paths."""
+    per path, then sweeps the record in segments of 8 vertices, last first: at 0 bounces (one
+    vertex), at 7 and 8 (one full segment, then one more vertex), at 17 (three segments) and at 64
+    (the most the path tracer takes) the gradient must equal the FP64 one over the same paths."""
     import torch
 
     hv, s0, _ = scene16
@@ -250,3 +251,52 @@ def test_gpu_odd_image_size(cq, oracle, gpu_ctx, scene, dscene, mode):
     img = cq.pathtrace(gpu_ctx, dscene, s, mode).cpu().numpy()
     ref = oracle.pathtrace(hv.nodes, hv.root, s.params)
     assert np.array_equal(img, ref), f"mode {mode}: {(img != ref).sum()} values differ"
+
+
+def _furnace_gradient(setup, alb, em, bounces):
+    """d(sum image)/d(albedo, emission) in the closed furnace: every pixel is e (1 + a + ... + a^B)
+    per channel, so the gradient is W H (1 + a + ... + a^B) for e and W H e (1 + 2a + ... + B a^(B-1))
+    for a."""
+    a = alb[0].astype(np.float64)
+    e = em[0].astype(np.float64)
+    px = setup.params.width * setup.params.height
+    de = px * sum(a ** k for k in range(bounces + 1))
+    da = px * e * sum(k * a ** (k - 1) for k in range(1, bounces + 1))
+    return da, de
+
+
+@pytest.mark.parametrize("bounces", [8, 17, 64])
+def test_oracle_adjoint_furnace_closed_form(cq, oracle, bounces):
+    """The FP64 adjoint of the oracle for paths of up to 65 vertices against the furnace's closed
+    form (every path bounces until the limit)."""
+    from tests.pt_pins import furnace_scene
+
+    hv, s, _ = furnace_scene(cq, W, bounces)
+    d_image = np.ones((s.params.height, s.params.width, 3), np.float64)
+    _, ra, re = oracle.pathtrace_f64(hv.nodes, hv.root, s.params, d_image)
+    da, de = _furnace_gradient(s, s.albedo, s.emission, bounces)
+    assert np.allclose(ra[0], da, rtol=1e-9) and np.allclose(re[0], de, rtol=1e-9), (ra, da, re, de)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bounces", [7, 8, 17, 64])
+def test_gpu_adjoint_furnace_deep_paths(cq, oracle, gpu_ctx, bounces):
+    """Every furnace path has bounces + 1 vertices, so the adjoint's backward sweep runs 1 to 9
+    segments of 8 vertices in every lane: the gradient equals the closed form and the FP64 oracle's
+    within 1e-4 relative."""
+    import torch
+
+    from tests.pt_pins import furnace_scene
+
+    hv, s, _ = furnace_scene(cq, W, bounces)
+    dv = gpu_ctx.upload(hv)
+    d_image = torch.ones((s.params.height, s.params.width, 3), dtype=torch.float32, device="cuda")
+    ga, ge = cq.pathtrace_adjoint(gpu_ctx, dv, s, d_image)
+    ga, ge = ga.cpu().numpy().astype(np.float64), ge.cpu().numpy().astype(np.float64)
+    _, ra, re = oracle.pathtrace_f64(hv.nodes, hv.root, s.params, d_image.cpu().numpy().astype(np.float64))
+    da, de = _furnace_gradient(s, s.albedo, s.emission, bounces)
+    for got, want, ref in ((ga[0], da, ra[0]), (ge[0], de, re[0])):
+        assert np.allclose(got, want, rtol=1e-4, atol=0), (bounces, got, want)
+        assert np.allclose(got, ref, rtol=1e-4, atol=0), (bounces, got, ref)
+    dv.free()
+    torch.cuda.synchronize()
