@@ -670,7 +670,10 @@ def main():
                         "HBM traffic ~0.3x the algorithmic bytes), bound by dependent child-read latency and "
                         "divergent VALU work (lane utilisation ~38 %), not by HBM bandwidth; 'hbm' names the "
                         "roofline the algorithmic bytes are priced against" if headline else
-                        "see DESIGN.md for the measured limiter at this size"),
+                        "measured at 16384^3 (tuned launch): ~1.6x the algorithmic bytes fetched (a 128-B line "
+                        "per 32-B node read, L2 hits ~52 %), at ~0.75 of HBM peak on that traffic: bound by HBM "
+                        "bandwidth on whole lines" if w is W.LARGE_16384 else
+                        "not profiled at this size"),
             "kernel_ms": round(avg_ms, 4),
             "bytes_per_launch": alg_bytes,
             "node_entries_per_ray": round(entries / args.rays, 3),
