@@ -69,7 +69,11 @@ LAUNCH_KEYS = ("variant", "block", "waves_per_cu", "regroup_interval", "refill_l
 
 
 def launch_string(cfg):
-    return ",".join(f"{k}={int(cfg[k])}" for k in LAUNCH_KEYS)
+    """The launch as `key=value` pairs; a short stack (stack_rows > 0) is appended, the full stack
+    (stack_rows 0, also a launch recorded before the field existed) adds nothing."""
+    s = ",".join(f"{k}={int(cfg[k])}" for k in LAUNCH_KEYS)
+    rows = int(cfg.get("stack_rows", 0) or 0)
+    return s + (f",stack_rows={rows}" if rows else "")
 
 
 def traffic_from_profiles(launch):
@@ -629,7 +633,7 @@ def main():
     if tuned:
         timed_launch = launch_string(tuned)
     elif args.launch:
-        timed_launch = launch_string({k: getattr(cfg, k) for k in LAUNCH_KEYS})
+        timed_launch = launch_string({k: getattr(cfg, k) for k in LAUNCH_KEYS + ("stack_rows",)})
     headline = w is W.PATHTRACE_1024 and args.rays == 1 << 24
     traffic, l2_hit_rate, profile = traffic_from_profiles(timed_launch) if headline else (None, None, None)
 
