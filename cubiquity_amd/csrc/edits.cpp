@@ -451,8 +451,33 @@ void plan_edits(const cqx_sphere_edit* edits, uint64_t n, int hs, EditPlan& plan
     const size_t nr = plan.keys.size();
     plan.morton_regions.resize(nr);
     for (size_t r = 0; r < nr; r++) plan.morton_regions[r] = uint32_t(r);
-    std::sort(plan.morton_regions.begin(), plan.morton_regions.end(),
-              [&](uint32_t a, uint32_t b) { return morton_less(plan.keys[a], plan.keys[b]); });
+    // Morton order: the interleaved code of the keys' 32 bits per axis, z above y above x at each
+    // bit (morton_less's rule: the axis of the most significant differing bit decides), sorted as
+    // 96-bit integers.
+    {
+        const auto spread = [](uint32_t v) { // bits 0..20 of v to bits 0, 3, ..., 60
+            uint64_t x = v & 0x1fffffu;
+            x = (x | (x << 32)) & 0x1f00000000ffffull;
+            x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+            x = (x | (x << 8)) & 0x100f00f00f00f00full;
+            x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+            x = (x | (x << 2)) & 0x1249249249249249ull;
+            return x;
+        };
+        struct Coded {
+            uint64_t hi, lo; // bits 21..31 and bits 0..20 of each axis, interleaved
+            uint32_t region;
+            bool operator<(const Coded& o) const { return hi != o.hi ? hi < o.hi : lo < o.lo; }
+        };
+        std::vector<Coded> code(nr);
+        for (size_t r = 0; r < nr; r++) {
+            const RegionKey& k = plan.keys[r];
+            code[r] = { spread(k.x >> 21) | (spread(k.y >> 21) << 1) | (spread(k.z >> 21) << 2),
+                        spread(k.x) | (spread(k.y) << 1) | (spread(k.z) << 2), uint32_t(r) };
+        }
+        std::sort(code.begin(), code.end());
+        for (size_t r = 0; r < nr; r++) plan.morton_regions[r] = code[r].region;
+    }
     const int levels = kRootHeight - hs;
     plan.level_keys.assign(size_t(levels) + 1, {});
     plan.level_children.assign(size_t(levels) + 1, {});
