@@ -337,21 +337,31 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 // segment only), then its vertices are swept from last to first.
 //
 // The gradient sums have a fixed order, so the result is the same bit for bit on every run: each
-// wave takes 64 consecutive paths at a time and, vertex slot by vertex slot, groups its lanes by
-// material (the first remaining lane's material, then the next); a group's six contributions are
-// summed over the wave by a fixed butterfly (lanes outside the group add 0) and lane 0 adds the
-// sums to the wave's own row of per-material totals in LDS. No atomics are involved; the block
-// adds its waves' rows in wave order.
+// wave takes 64 consecutive paths at a time. With at most kAccMaterials materials (kAcc), each
+// lane adds its paths' contributions, vertex by vertex in order, into its own registers, one row per
+// material, and after its last path the wave sums each row by a fixed butterfly into its row of
+// per-material totals in LDS. With more materials, vertex slot by vertex slot, the wave groups its
+// lanes by material (the first remaining lane's material, then the next); a group's six
+// contributions are summed over the wave by a fixed butterfly (lanes outside the group add 0) and
+// lane 0 adds the sums to the wave's row. No atomics are involved; the block adds its waves' rows
+// in wave order.
 constexpr uint32_t kAdjointBlock = 256;
 constexpr int kSegment = 8;
 constexpr uint32_t kAdjointWaves = kAdjointBlock / 64;
+constexpr int kAccMaterials = 16;
 
+template <bool kAcc>
 __global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams* __restrict__ dp, uint64_t P,
                                                               const uint8_t* __restrict__ vmat,
                                                               const uint8_t* __restrict__ vstat,
                                                               const float* __restrict__ dimage, uint32_t M,
                                                               float* __restrict__ partials)
 {
+    float acc[kAcc ? kAccMaterials : 1][6]; // kAcc: this lane's sums, row m - 1 for material m
+#pragma unroll
+    for (int mm = 0; mm < (kAcc ? kAccMaterials : 1); mm++)
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[mm][k] = 0.0f;
     __shared__ float g[kAdjointWaves][256 * 6]; // per wave: [material][dA rgb, dE rgb]
     for (uint32_t k = threadIdx.x; k < kAdjointWaves * 256 * 6; k += blockDim.x) (&g[0][0])[k] = 0.0f;
     __syncthreads();
@@ -413,6 +423,17 @@ __global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams*
                         G[a] = d.emission[m * 3 + a] + d.albedo[m * 3 + a] * G[a];
                     }
                 }
+                if constexpr (kAcc) {
+                    if (live) {
+#pragma unroll
+                        for (int mm = 0; mm < kAccMaterials; mm++) {
+                            const bool hit = m == uint32_t(mm + 1);
+#pragma unroll
+                            for (int k = 0; k < 6; k++) acc[mm][k] += hit ? c[k] : 0.0f;
+                        }
+                    }
+                    continue;
+                }
                 uint64_t todo = __ballot(live);
                 while (todo) {
                     const uint32_t m0 = uint32_t(__shfl(int(m), __ffsll((unsigned long long)todo) - 1));
@@ -428,6 +449,17 @@ __global__ void __launch_bounds__(kAdjointBlock) k_pt_adjoint(const PtDevParams*
                 }
             }
         }
+    }
+    if constexpr (kAcc) {
+#pragma unroll
+        for (int mm = 0; mm < kAccMaterials; mm++)
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                float v = acc[mm][k];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0) gw[(mm + 1) * 6 + k] = v;
+            }
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < M * 6; k += blockDim.x) { // material ids 1..M
@@ -775,8 +807,12 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     }
     {
         const cqx::TraceRange sweep("adjoint: backward sweep and reduction");
-        hipLaunchKernelGGL(k_pt_adjoint, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat, d_image,
-                           pp->n_materials, p->partials);
+        if (pp->n_materials <= uint32_t(kAccMaterials))
+            hipLaunchKernelGGL(k_pt_adjoint<true>, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat,
+                               d_image, pp->n_materials, p->partials);
+        else
+            hipLaunchKernelGGL(k_pt_adjoint<false>, dim3(blocks), dim3(block), 0, s, p->dparams, p->P, p->vmat, p->vstat,
+                               d_image, pp->n_materials, p->partials);
         hipLaunchKernelGGL(k_pt_reduce, dim3(pp->n_materials * 6), dim3(256), 0, s, p->partials, blocks,
                            pp->n_materials, d_albedo, d_emission);
     }
