@@ -333,9 +333,15 @@ __global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64
 // cell order through `order`, and as each XCD drains its own contiguous part of the id space, its
 // rays in flight come from a few neighbouring cells. The order within a cell is whatever the
 // atomics give: every ray's record is independent of it.
+// A counting sort without global atomics: each of up to kMaxBinBlocks workgroups counts the cells
+// of its contiguous range of rays in LDS and writes its histogram; one thread per cell turns the
+// histograms into each workgroup's offset within the cell and the cell's total; the totals are
+// scanned; each workgroup then places its rays from its own offsets (LDS atomics only).
 constexpr uint32_t kMaxBinBits = 4;
 constexpr uint32_t kMaxCells = 1u << (3 * kMaxBinBits); // 4096
-constexpr uint32_t kBinTile = 16384;                    // rays per scatter tile
+constexpr uint32_t kMaxBinBlocks = 256;
+constexpr uint32_t kBinBlock = 1024;   // threads per workgroup of the count and scatter kernels
+constexpr uint32_t kBinMinRange = 16384; // rays per workgroup at least
 
 struct BinGrid {
     float lo[3];
@@ -358,27 +364,39 @@ __device__ __forceinline__ uint32_t bin_cell(const BinGrid& g, const float* r)
     return c; // < 2^(3 bits)
 }
 
-__global__ void k_zero_u32(uint32_t* __restrict__ p, uint32_t count)
+// Cell of every ray of the workgroup's range [blockIdx.x * range, ...), and the range's cell
+// histogram, hist[blockIdx.x * ncells + cell].
+__global__ void __launch_bounds__(kBinBlock) k_bin_count(const cqx_ray* __restrict__ rays, uint64_t n, uint64_t range,
+                                                         BinGrid g, uint32_t ncells, uint16_t* __restrict__ cells,
+                                                         uint32_t* __restrict__ hist)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0u;
-}
-
-// Cell of every ray, and the number of rays per cell (counts must start at zero).
-__global__ void __launch_bounds__(256) k_bin_count(const cqx_ray* __restrict__ rays, uint64_t n, BinGrid g,
-                                                   uint32_t ncells, uint16_t* __restrict__ cells,
-                                                   uint32_t* __restrict__ counts)
-{
-    __shared__ uint32_t hist[kMaxCells];
-    for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x) hist[b] = 0u;
+    __shared__ uint32_t h[kMaxCells];
+    for (uint32_t c = threadIdx.x; c < ncells; c += blockDim.x) h[c] = 0u;
     __syncthreads();
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t first = uint64_t(blockIdx.x) * range, last = first + range < n ? first + range : n;
+    for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x) {
         const uint32_t c = bin_cell(g, reinterpret_cast<const float*>(rays + i));
         cells[i] = uint16_t(c);
-        atomicAdd(&hist[c], 1u);
+        atomicAdd(&h[c], 1u);
     }
     __syncthreads();
-    for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x)
-        if (hist[b]) atomicAdd(&counts[b], hist[b]);
+    for (uint32_t c = threadIdx.x; c < ncells; c += blockDim.x) hist[uint64_t(blockIdx.x) * ncells + c] = h[c];
+}
+
+// Per cell (one thread each): the workgroups' counts become their offsets within the cell, in
+// workgroup order, and totals[cell] the cell's count.
+__global__ void __launch_bounds__(256) k_bin_columns(uint32_t* __restrict__ hist, uint32_t groups, uint32_t ncells,
+                                                     uint32_t* __restrict__ totals)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < groups; b++) {
+        const uint32_t v = hist[uint64_t(b) * ncells + c];
+        hist[uint64_t(b) * ncells + c] = run;
+        run += v;
+    }
+    totals[c] = run;
 }
 
 // Exclusive prefix sum of counts[0, ncells) in place (ncells <= 4096: four per thread).
@@ -410,29 +428,20 @@ __global__ void __launch_bounds__(1024) k_bin_scan(uint32_t* __restrict__ counts
     }
 }
 
-// order[offsets[c] ...] = the ids of the rays in cell c. Per tile of kBinTile rays: a histogram in
-// LDS, one global reservation per cell present, then each ray takes the next slot of its cell.
-// The reservations of all tiles for cell c add up to its count, so every slot of [0, n) is
-// written once.
-__global__ void __launch_bounds__(256) k_bin_scatter(const uint16_t* __restrict__ cells, uint64_t n, uint32_t ncells,
-                                                     uint32_t* __restrict__ offsets, uint32_t* __restrict__ order)
+// order[...] = the ids of the rays of cell c, from cell c's first slot (base[c]) on, each
+// workgroup's rays of the cell from its offset (hist) within the cell on. The offsets of the
+// workgroups of cell c add up to its count, so every slot of [0, n) is written once.
+__global__ void __launch_bounds__(kBinBlock) k_bin_scatter(const uint16_t* __restrict__ cells, uint64_t n,
+                                                           uint64_t range, uint32_t ncells,
+                                                           const uint32_t* __restrict__ base,
+                                                           const uint32_t* __restrict__ hist,
+                                                           uint32_t* __restrict__ order)
 {
     __shared__ uint32_t cur[kMaxCells];
-    const uint64_t tiles = (n + kBinTile - 1) / kBinTile;
-    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const uint64_t first = tile * kBinTile;
-        const uint64_t last = first + kBinTile < n ? first + kBinTile : n;
-        for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x) cur[b] = 0u;
-        __syncthreads();
-        for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x) atomicAdd(&cur[cells[i]], 1u);
-        __syncthreads();
-        for (uint32_t b = threadIdx.x; b < ncells; b += blockDim.x)
-            if (cur[b]) cur[b] = atomicAdd(&offsets[b], cur[b]);
-        __syncthreads();
-        for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x)
-            order[atomicAdd(&cur[cells[i]], 1u)] = uint32_t(i);
-        __syncthreads();
-    }
+    for (uint32_t c = threadIdx.x; c < ncells; c += blockDim.x) cur[c] = base[c] + hist[uint64_t(blockIdx.x) * ncells + c];
+    __syncthreads();
+    const uint64_t first = uint64_t(blockIdx.x) * range, last = first + range < n ? first + range : n;
+    for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x) order[atomicAdd(&cur[cells[i]], 1u)] = uint32_t(i);
 }
 
 // The binning grid: 2^bits cells per axis over the union of the boxes of the non-empty SubDAGs.
@@ -460,7 +469,8 @@ bool bin_grid(const cqx_dvolume* vol, uint32_t bits, BinGrid& g)
     return true;
 }
 
-// Binning buffers of the context: kMaxCells counts, then n order entries (u32) and n cells (u16).
+// Binning buffers: kMaxCells counts, kMaxBinBlocks x kMaxCells histogram entries, then n order
+// entries (u32) and n cells (u16).
 size_t bin_bytes(uint64_t n);
 
 cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
@@ -479,7 +489,10 @@ cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
 // Runs the binning prepass of n rays on `stream`; `order` receives the device pointer of the
 // resulting id -> ray map. The buffers are the context's (grown as needed) or, with `sv`, the
 // caller-provided scratch, which must hold bin_bytes(n).
-size_t bin_bytes(uint64_t n) { return size_t(kMaxCells) * 4 + size_t(n) * 4 + size_t(n) * 2; }
+size_t bin_bytes(uint64_t n)
+{
+    return size_t(kMaxCells) * 4 + size_t(kMaxBinBlocks) * kMaxCells * 4 + size_t(n) * 4 + size_t(n) * 2;
+}
 
 cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid& g, hipStream_t stream,
                     const cqx::ScratchView* sv, const uint32_t*& order)
@@ -495,16 +508,19 @@ cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid
         bins = ctx->bins;
     }
     uint32_t* counts = static_cast<uint32_t*>(bins);
-    uint32_t* ord = counts + kMaxCells;
+    uint32_t* hist = counts + kMaxCells;
+    uint32_t* ord = hist + size_t(kMaxBinBlocks) * kMaxCells;
     uint16_t* cells = reinterpret_cast<uint16_t*>(ord + n);
     const uint32_t ncells = 1u << (3 * g.bits);
-    const uint64_t grid_cap = uint64_t(ctx->cu_count) * 4;
-    const uint32_t count_blocks = uint32_t(std::max<uint64_t>(1, std::min(grid_cap, (n + 255) / 256)));
-    const uint32_t scatter_blocks = uint32_t(std::max<uint64_t>(1, std::min(grid_cap, (n + kBinTile - 1) / kBinTile)));
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, stream, counts, ncells);
-    hipLaunchKernelGGL(k_bin_count, dim3(count_blocks), dim3(256), 0, stream, rays, n, g, ncells, cells, counts);
+    const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>(kMaxBinBlocks, n / kBinMinRange));
+    const uint64_t range = (n + groups - 1) / groups;
+    hipLaunchKernelGGL(k_bin_count, dim3(uint32_t(groups)), dim3(kBinBlock), 0, stream, rays, n, range, g, ncells, cells,
+                       hist);
+    hipLaunchKernelGGL(k_bin_columns, dim3((ncells + 255) / 256), dim3(256), 0, stream, hist, uint32_t(groups), ncells,
+                       counts);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, stream, counts, ncells);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(scatter_blocks), dim3(256), 0, stream, cells, n, ncells, counts, ord);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(uint32_t(groups)), dim3(kBinBlock), 0, stream, cells, n, range, ncells, counts,
+                       hist, ord);
     HIP_TRY(hipGetLastError());
     order = ord;
     return CQX_OK;
@@ -548,7 +564,8 @@ cqx_status check_ctx(cqx_ctx* ctx)
 // hipGetSymbolAddress returns with that device current (kept per device in noalloc_scratch_ptr),
 // never through the array's name, which on the host is a stand-in and not a device address.
 constexpr size_t kNoallocCounterBytes = 1024;
-constexpr size_t kNoallocBinBytes = size_t(kMaxCells) * 4 + size_t(CQX_NOALLOC_MAX_BINNED_RAYS) * 6;
+constexpr size_t kNoallocBinBytes =
+    size_t(kMaxCells) * 4 + size_t(kMaxBinBlocks) * kMaxCells * 4 + size_t(CQX_NOALLOC_MAX_BINNED_RAYS) * 6;
 static_assert(kNoallocCounterBytes + kNoallocBinBytes == CQX_NOALLOC_SCRATCH_BYTES, "cqx.h states the scratch size");
 static_assert(sizeof(unsigned long long) * kQueues * kCounterStride <= kNoallocCounterBytes, "counters fit");
 static __device__ __align__(256) unsigned char scratch[kNoallocCounterBytes + kNoallocBinBytes];

@@ -349,7 +349,7 @@ cqx_status cqx_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ra
  * "auto" launch bins) takes at most CQX_NOALLOC_MAX_BINNED_RAYS rays; more fail with
  * CQX_ERR_LIMIT and nothing is launched. */
 #define CQX_NOALLOC_MAX_BINNED_RAYS (1u << 22)
-#define CQX_NOALLOC_SCRATCH_BYTES (1024u + 4096u * 4u + 6u * CQX_NOALLOC_MAX_BINNED_RAYS)
+#define CQX_NOALLOC_SCRATCH_BYTES (1024u + 4096u * 4u + 256u * 4096u * 4u + 6u * CQX_NOALLOC_MAX_BINNED_RAYS)
 cqx_status cqx_intersect_noalloc(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* rays, uint64_t n,
                                  uint32_t flags, float max_footprint, cqx_hit* hits,
                                  const cqx_launch_cfg* cfg, void* stream);
