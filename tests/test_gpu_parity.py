@@ -66,11 +66,12 @@ def test_gpu_binned_matches_oracle_128(cq, oracle, gpu_ctx, dvol128, vol128, ray
 
 
 def test_gpu_binned_ragged_counts(cq, gpu_ctx, dvol128, vol128):
-    """Binned launches of 1 ray, a few rays, and counts either side of the 16384-ray scatter tile,
-    against unbinned launches; then a smaller batch after a larger one (stale order entries)."""
-    rays = cq.incoherent_rays(vol128, 7, 11, 40000)
+    """Binned launches of 1 ray, a few rays, and counts either side of where the prepass splits the
+    rays into two and three workgroup ranges (16384 rays per range at least), against unbinned
+    launches; then a smaller batch after a larger one (stale order entries)."""
+    rays = cq.incoherent_rays(vol128, 7, 11, 50000)
     full = _gpu(gpu_ctx, dvol128, rays, True, -1.0, cq.LaunchCfg(variant=2))
-    for n in (40000, 1, 17, 16383, 16384, 16385, 32769):
+    for n in (50000, 1, 17, 16383, 16384, 16385, 32767, 32768, 32769, 49151, 49152, 49153):
         g = _gpu(gpu_ctx, dvol128, rays[:n], True, -1.0, cq.LaunchCfg(variant=2, bin_bits=3))
         assert records_equal(g, full[:n]).all(), n
 
