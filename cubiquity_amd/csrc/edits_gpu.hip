@@ -551,6 +551,62 @@ __global__ void __launch_bounds__(1024) k_scan_u64(uint64_t* v, uint64_t n, uint
     block_exclusive_scan_u64(v, n, total);
 }
 
+// Exclusive prefix sums over many workgroups: tiles of kScanTile values, 4 consecutive values per
+// thread. k_scan_tile_sums sums each tile, the tile sums are scanned in one workgroup, and
+// k_scan_tiles scans each tile from its tile's offset.
+constexpr uint32_t kScanBlock = 1024, kScanTile = 4 * kScanBlock;
+
+__device__ uint64_t block_sum_u64(uint64_t s)
+{
+    __shared__ uint64_t part[kScanBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    uint64_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) t += part[w];
+    return t;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_tile_sums(const uint64_t* __restrict__ v, uint64_t n,
+                                                               uint64_t* __restrict__ sums)
+{
+    const uint64_t first = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * 4;
+    uint64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += first + k < n ? v[first + k] : 0u;
+    const uint64_t t = block_sum_u64(s);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_tiles(uint64_t* __restrict__ v, uint64_t n,
+                                                           const uint64_t* __restrict__ offsets)
+{
+    __shared__ uint64_t part[kScanBlock];
+    const uint64_t first = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * 4;
+    uint64_t x[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        x[k] = first + k < n ? v[first + k] : 0u;
+        s += x[k];
+    }
+    const uint32_t t = threadIdx.x;
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t d = 1; d < blockDim.x; d <<= 1) {
+        const uint64_t y = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    uint64_t run = offsets[blockIdx.x] + part[t] - s;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (first + k < n) {
+            v[first + k] = run;
+            run += x[k];
+        }
+}
+
 __global__ void __launch_bounds__(256) k_counts_to_u64(const uint32_t* __restrict__ c, uint64_t* __restrict__ o, uint32_t n)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) o[i] = c[i];
@@ -800,7 +856,7 @@ enum : int {
     kBufEdits, kBufKeys, kBufFirst, kBufEditIds, kBufRun, kBufPoolOff, kBufCap, kBufCounts, kBufResults,
     kBufOverflow, kBufPool, kBufTable, kBufOff, kBufSrc, kBufRegion, kBufLevel, kBufKey, kBufSlotOf, kBufRep,
     kBufMergeTable, kBufFin, kBufScalars, kBufLvlOff, kBufTopKeys, kBufCbeg, kBufCend, kBufMorton, kBufRes,
-    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot, kBufNext
+    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot, kBufNext, kBufScanSums
 };
 
 // Buffer `slot` of the volume's edit state with at least `bytes` (contents not kept when grown).
@@ -820,12 +876,33 @@ cqx_status buf(cqx::DeviceEdits* E, int slot, size_t count, T** out)
     return CQX_OK;
 }
 
+// Exclusive prefix sums of v[0, n) in place and their total in *total, on the null stream.
+cqx_status scan_u64(cqx_ctx* ctx, cqx::DeviceEdits* E, uint64_t* v, uint64_t n, uint64_t* total);
+
 template <typename T>
 cqx_status upload(cqx::DeviceEdits* E, int slot, const std::vector<T>& v, T** out)
 {
     cqx_status st = buf(E, slot, v.size(), out);
     if (st != CQX_OK) return st;
     if (!v.empty()) HIP_TRY(hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return CQX_OK;
+}
+
+cqx_status scan_u64(cqx_ctx*, cqx::DeviceEdits* E, uint64_t* v, uint64_t n, uint64_t* total)
+{
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles <= 1) {
+        hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, 0, v, n, total);
+    } else {
+        if (tiles > 0x7fffffffull) return fail(CQX_ERR_LIMIT, "scan of %llu values", (unsigned long long)n);
+        uint64_t* sums = nullptr;
+        cqx_status st = buf(E, kBufScanSums, tiles, &sums);
+        if (st != CQX_OK) return st;
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(uint32_t(tiles)), dim3(kScanBlock), 0, 0, v, n, sums);
+        hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, 0, sums, tiles, total);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(uint32_t(tiles)), dim3(kScanBlock), 0, 0, v, n, sums);
+    }
+    HIP_TRY(hipGetLastError());
     return CQX_OK;
 }
 
@@ -1064,7 +1141,7 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
     uint8_t* d_level;
     if ((st = buf(E, kBufOff, nr + 1, &d_off)) != CQX_OK || (st = buf(E, kBufScalars, 8, &d_scalars)) != CQX_OK) return st;
     hipLaunchKernelGGL(k_counts_to_u64, dim3(grid_for(nr, ctx, 256)), dim3(256), 0, 0, d_counts, d_off, nr);
-    hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, 0, d_off, uint64_t(nr), d_scalars);
+    if ((st = scan_u64(ctx, E, d_off, uint64_t(nr), d_scalars)) != CQX_OK) return st;
     uint64_t total = 0;
     HIP_TRY(hipMemcpy(&total, d_scalars, 8, hipMemcpyDeviceToHost));
     if (base + total >= 0xfffffffeull) return fail(CQX_ERR_LIMIT, "store too large for on-device edits");
@@ -1091,7 +1168,7 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
             hipLaunchKernelGGL(k_merge_rep, dim3(g), dim3(256), 0, 0, M, lv);
         }
         hipLaunchKernelGGL(k_merge_kept, dim3(g), dim3(256), 0, 0, M, d_fin);
-        hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, 0, d_fin, total, d_scalars + 2);
+        if ((st = scan_u64(ctx, E, d_fin, total, d_scalars + 2)) != CQX_OK) return st;
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpy(&kept_total, d_scalars + 2, 8, hipMemcpyDeviceToHost));
     }
