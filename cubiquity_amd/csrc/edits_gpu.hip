@@ -322,6 +322,9 @@ __device__ bool enter(const Region& R, const Brush& b, Frame& f, uint32_t node, 
     return true;
 }
 
+// A value that is the same in every active lane, as a scalar.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // EditWalk::fill (edits.cpp) without recursion, on the whole wave; returns the new node or
 // kOverflow (the same value in every lane).
 __device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0, uint32_t height0, int32_t lx,
@@ -332,11 +335,12 @@ __device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0
     bool returned = false;
     uint32_t ret = 0;
     for (;;) {
+        // the frame's fields are the same in every lane: read as scalars (uniform branches)
         Frame& f = st[sp];
-        uint32_t next = f.next, changed = f.changed;
+        uint32_t next = uni(f.next), changed = uni(f.changed);
         if (returned) { // the child frame of position `next` came back with `ret`
             if (ret == kOverflow) return kOverflow;
-            if (ret != f.out[next]) {
+            if (ret != uni(f.out[next])) {
                 if (lane == 0) f.out[next] = ret;
                 changed = 1;
             }
@@ -344,18 +348,20 @@ __device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0
             returned = false;
         }
         bool pushed = false;
-        const uint32_t side = 1u << (f.height - 1);
+        const uint32_t height = uni(f.height), okm = uni(f.ok), outm = uni(f.outside), inm = uni(f.inside);
+        const uint32_t side = 1u << (height - 1);
         for (; next < 8; next++) {
-            if (!((f.ok >> next) & 1u)) continue;
-            const uint32_t child = f.out[next];
+            if (!((okm >> next) & 1u)) continue;
+            const uint32_t child = uni(f.out[next]);
             if (child == mat) continue;
-            if ((f.outside >> next) & 1u) continue; // set only at height >= 2
-            const bool all = (f.inside >> next) & 1u;
-            if (f.height >= 2 && !all) {
+            if ((outm >> next) & 1u) continue; // set only at height >= 2
+            const bool all = (inm >> next) & 1u;
+            if (height >= 2 && !all) {
                 const uint32_t cx = next & 1u, cy = (next >> 1) & 1u, cz = next >> 2;
-                const int32_t x0 = int32_t(uint32_t(f.lo[0]) + side * cx), y0 = int32_t(uint32_t(f.lo[1]) + side * cy),
-                              z0 = int32_t(uint32_t(f.lo[2]) + side * cz);
-                if (enter(R, b, st[sp + 1], child, f.height - 1, x0, y0, z0, lane)) {
+                const int32_t x0 = int32_t(uni(uint32_t(f.lo[0])) + side * cx),
+                              y0 = int32_t(uni(uint32_t(f.lo[1])) + side * cy),
+                              z0 = int32_t(uni(uint32_t(f.lo[2])) + side * cz);
+                if (enter(R, b, st[sp + 1], child, height - 1, x0, y0, z0, lane)) {
                     if (lane == 0) {
                         f.next = next;
                         f.changed = changed;
@@ -373,12 +379,12 @@ __device__ uint32_t fill(Region& R, const Brush& b, uint32_t mat, uint32_t node0
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (pushed) continue;
-        uint32_t res = f.node;
+        uint32_t res = uni(f.node);
         if (changed) {
             DNode n;
 #pragma unroll
-            for (int c = 0; c < 8; c++) n.c[c] = f.out[c];
-            res = R.intern(n);
+            for (int c = 0; c < 8; c++) n.c[c] = uni(f.out[c]);
+            res = uni(R.intern(n));
         }
         if (sp == 0 || res == kOverflow) return res;
         sp--;
@@ -661,22 +667,38 @@ struct TopArgs {
     const uint64_t* tab_off;
     uint32_t* out_count;  // in: store size after the merge; out: after the top levels
     uint32_t* out_root;
+    uint32_t* old;        // per entry of level >= 1: the node it held before the edits
 };
 
 __global__ void __launch_bounds__(kTopBlock) k_edit_top(TopArgs A)
 {
-    __shared__ uint32_t part[kTopBlock];
+    __shared__ uint32_t wave_kept[kTopBlock / 64];
     __shared__ uint32_t count_s;
     const uint32_t t = threadIdx.x, T = blockDim.x;
     if (t == 0) count_s = *A.out_count;
     for (uint32_t i = A.lvl_off[0] + t; i < A.lvl_off[1]; i += T) A.res[i] = A.results[A.morton_regions[i]];
+    // The nodes the entries held before the edits, top down: the top entry (height 32, one key)
+    // holds the root, and an entry's children hold the children of its node (node_at of every
+    // entry, with one read per entry instead of one per level above it).
+    if (t == 0) A.old[A.lvl_off[A.levels]] = A.root;
     __threadfence();
     __syncthreads();
+    for (uint32_t l = A.levels; l >= 2; l--) {
+        for (uint32_t i = A.lvl_off[l] + t; i < A.lvl_off[l + 1]; i += T) {
+            const uint32_t p = __hip_atomic_load(A.old + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (uint32_t j = A.cbeg[i]; j < A.cend[i]; j++) {
+                const DRegionKey k = A.keys[j];
+                const uint32_t id = (k.x & 1u) | (k.y & 1u) << 1 | (k.z & 1u) << 2;
+                A.old[j] = p < kMat ? p : A.nodes[uint64_t(p) * 8 + id];
+            }
+        }
+        __threadfence();
+        __syncthreads();
+    }
     for (uint32_t l = 1; l <= A.levels; l++) {
         const uint32_t b = A.lvl_off[l], e = A.lvl_off[l + 1];
-        const int h = A.hs + int(l);
         for (uint32_t i = b + t; i < e; i += T) {
-            const uint32_t old = node_at(A.nodes, A.root, h, A.keys[i]);
+            const uint32_t old = __hip_atomic_load(A.old + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             DNode out;
             for (int c = 0; c < 8; c++) out.c[c] = old < kMat ? old : A.nodes[uint64_t(old) * 8 + c];
             bool changed = false;
@@ -727,25 +749,27 @@ __global__ void __launch_bounds__(kTopBlock) k_edit_top(TopArgs A)
             if (A.cand[i]) A.crep[i] = __hip_atomic_load(tab + A.cslot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         __syncthreads();
-        // the kept ones appended in order: a scan over the level in chunks of T entries
+        // the kept ones appended in order: a scan over the level in chunks of T entries (each wave
+        // ranks its lanes by ballot, then adds the counts of the waves before it)
         uint32_t base = count_s;
         for (uint32_t c0 = b; c0 < e; c0 += T) {
             const uint32_t i = c0 + t;
-            const uint32_t keep = (i < e && A.cand[i] && A.crep[i] == i) ? 1u : 0u;
-            part[t] = keep;
+            const bool keep = i < e && A.cand[i] && A.crep[i] == i;
+            const uint64_t bal = __ballot(keep);
+            const uint32_t lane = t & 63u, w = t >> 6;
+            if (lane == 0) wave_kept[w] = uint32_t(__popcll(bal));
             __syncthreads();
-            for (uint32_t d = 1; d < T; d <<= 1) {
-                const uint32_t x = t >= d ? part[t - d] : 0u;
-                __syncthreads();
-                part[t] += x;
-                __syncthreads();
+            uint32_t before = 0, all = 0;
+            for (uint32_t v = 0; v < T / 64; v++) {
+                before += v < w ? wave_kept[v] : 0u;
+                all += wave_kept[v];
             }
             if (keep) {
-                const uint32_t at = base + part[t] - 1u;
+                const uint32_t at = base + before + uint32_t(__popcll(bal & ((1ull << lane) - 1ull)));
                 store_node(A.nodes, at, load_node_coherent(A.ckey, i));
                 A.res[i] = at; // a kept entry's node; the others copy theirs below
             }
-            base += part[T - 1];
+            base += all;
             __syncthreads();
         }
         __threadfence();
@@ -856,7 +880,7 @@ enum : int {
     kBufEdits, kBufKeys, kBufFirst, kBufEditIds, kBufRun, kBufPoolOff, kBufCap, kBufCounts, kBufResults,
     kBufOverflow, kBufPool, kBufTable, kBufOff, kBufSrc, kBufRegion, kBufLevel, kBufKey, kBufSlotOf, kBufRep,
     kBufMergeTable, kBufFin, kBufScalars, kBufLvlOff, kBufTopKeys, kBufCbeg, kBufCend, kBufMorton, kBufRes,
-    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot, kBufNext, kBufScanSums
+    kBufCand, kBufCkey, kBufTopTable, kBufRoot, kBufCslot, kBufTabOff, kBufSubRoot, kBufNext, kBufScanSums, kBufOld
 };
 
 // Buffer `slot` of the volume's edit state with at least `bytes` (contents not kept when grown).
@@ -1224,13 +1248,15 @@ cqx_status cqx_volume_edit_spheres(cqx_ctx* ctx, cqx_dvolume* vol, const cqx_sph
         (st = buf(E, kBufRoot, 2, &d_root)) != CQX_OK)
         return st;
     if ((st = buf(E, kBufCslot, ntop, &d_cslot)) != CQX_OK) return st;
+    uint32_t* d_old = nullptr;
+    if ((st = buf(E, kBufOld, ntop, &d_old)) != CQX_OK) return st;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(tab_off[levels + 1], ctx, 256)), dim3(256), 0, 0, d_ttable,
                        tab_off[levels + 1], kEmptySlot);
     const uint32_t count_after_merge = uint32_t(base + kept_total);
     HIP_TRY(hipMemcpy(d_root, &count_after_merge, 4, hipMemcpyHostToDevice));
     TopArgs T{ vol->nodes, E->index, E->index_cap - 1, vol->root, hs, levels, d_lvl_off, d_tkeys, d_cbeg, d_cend,
                d_morton, d_results, d_res, d_cand, d_ckey, d_cand + ntop, d_cslot, d_ttable, d_tab_off, d_root,
-               d_root + 1 };
+               d_root + 1, d_old };
     hipLaunchKernelGGL(k_edit_top, dim3(1), dim3(kTopBlock), 0, 0, T);
     HIP_TRY(hipGetLastError());
     uint32_t out[2] = { 0, 0 };
