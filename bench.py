@@ -91,6 +91,24 @@ def traffic_from_profiles(launch):
     return None, None, None
 
 
+def profiled_launches():
+    """Launch strings of the committed traffic profile of the headline traversal."""
+    path = os.path.join(ROOT, "profiles", "traffic_intersect_incoherent_1024.json")
+    if not os.path.exists(path):
+        return []
+    with open(path) as f:
+        return [p["launch"] for p in json.load(f).get("launches", []) if "launch" in p]
+
+
+def launch_cfg(cq, launch):
+    """LaunchCfg of a launch string."""
+    cfg = cq.LaunchCfg()
+    for kv in launch.split(","):
+        k, v = kv.split("=")
+        setattr(cfg, k.strip(), int(v))
+    return cfg
+
+
 def cpu_baseline(hv, rays, hits_gpu, cq):
     """Reference CPU raycast on the host cores over a bounded sample of the same rays, with the
     same sample also timed on one thread (the per-core rate, for scaling to other core counts)."""
@@ -606,6 +624,24 @@ def main():
         tuned["tuning_s"] = round(time.time() - t0, 2)
         tuned["tuned_ms"] = round(best_ms, 4)
         log(f"autotuned launch: {tuned}")
+        # The autotuner's choice among near-equal launches varies from box to box. When it has no
+        # committed counters and a launch that has them is as fast (within 0.5 %, both re-timed
+        # here), that launch is the one timed, so that the line's traffic figures are of the launch
+        # it measured.
+        if (w is W.PATHTRACE_1024 and args.rays == 1 << 24 and
+                traffic_from_profiles(launch_string(tuned))[0] is None):
+            run = lambda c: cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=c, out=d_hits)  # noqa: E731
+            t_auto = min(timed(lambda: run(cfg), 5) for _ in range(3))
+            alternatives = []
+            for s_launch in profiled_launches():
+                c = launch_cfg(cq, s_launch)
+                alternatives.append((min(timed(lambda: run(c), 5) for _ in range(3)), s_launch, c))
+            alternatives.sort(key=lambda a: a[0])
+            if alternatives and alternatives[0][0] <= t_auto * 1.005:
+                t_alt, s_launch, cfg = alternatives[0]
+                tuned["timed_instead"] = {"launch": s_launch, "ms": round(t_alt, 4), "tuned_launch_ms": round(t_auto, 4),
+                                          "reason": "profiled launch within 0.5 % of the autotuned one"}
+                log(f"timing the profiled launch {s_launch} ({t_alt:.4f} ms) instead of the tuned one ({t_auto:.4f} ms)")
     for _ in range(args.warmup):
         cq.intersectVolume(ctx, dvol, d_rays, True, -1.0, cfg=cfg, out=d_hits)
     torch.cuda.synchronize()
@@ -630,7 +666,9 @@ def main():
     alg_bytes = args.rays * (24 + 32) + 32 * entries
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     timed_launch = None
-    if tuned:
+    if tuned and "timed_instead" in tuned:
+        timed_launch = tuned["timed_instead"]["launch"]
+    elif tuned:
         timed_launch = launch_string(tuned)
     elif args.launch:
         timed_launch = launch_string({k: getattr(cfg, k) for k in LAUNCH_KEYS + ("stack_rows",)})
