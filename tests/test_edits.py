@@ -279,3 +279,23 @@ def test_region_merge_store_is_pinned(cq):
             hv.edit_spheres(e, threads=8)
         h = hashlib.sha256(np.ascontiguousarray(hv.nodes).tobytes() + str(hv.root).encode()).hexdigest()
         assert h[:16] == digest, log2
+
+
+def test_edit_plan_against_its_definition(tmp_path):
+    """tests/cpp/plan_check.cpp: the region plan of random sphere batches (around the origin, where
+    the region keys cross a power-of-two boundary, far from it, and spread wide) against its
+    definition: touched regions with their spheres in order, ascending keys, level-0 entries in the
+    Morton order of the pairwise rule, each level the distinct parents of the one below."""
+    import glob
+    import os
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    csrc = os.path.join(root, "cubiquity_amd", "csrc")
+    exe = str(tmp_path / "plan_check")
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", f"-I{os.path.join(root, 'include')}", f"-I{csrc}",
+                        os.path.join(root, "tests", "cpp", "plan_check.cpp")] + sorted(glob.glob(os.path.join(csrc, "*.cpp"))) +
+                       ["-o", exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "plan ok" in r.stdout, r.stdout + r.stderr
