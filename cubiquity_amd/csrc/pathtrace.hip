@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 // lane 0 adds the sums to the wave's row. No atomics are involved; the block adds its waves' rows
 // in wave order.
 constexpr uint32_t kAdjointBlock = 256;
-constexpr int kSegment = 8;
+constexpr int kSegment = 4; // vertices per segment: a path of up to 3 bounces is one
 constexpr uint32_t kAdjointWaves = kAdjointBlock / 64;
 constexpr int kAccMaterials = 16;
 

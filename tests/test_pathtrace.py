@@ -207,8 +207,8 @@ def test_gpu_adjoint_is_bit_reproducible(cq, gpu_ctx, scene16):
 @pytest.mark.parametrize("bounces", [0, 7, 8, 17, 64])
 def test_gpu_adjoint_at_bounce_limits(cq, oracle, gpu_ctx, scene16, bounces):
     """The adjoint replays the wavefront pass, which records up to bounces + 1 vertex materials
-    per path, then sweeps the record in segments of 8 vertices, last first: at 0 bounces (one
-    vertex), at 7 and 8 (one full segment, then one more vertex), at 17 (three segments) and at 64
+    per path, then sweeps the record in segments of 4 vertices, last first: at 0 bounces (one
+    vertex), at 7 and 8 (two full segments, then one more vertex), at 17 (five segments) and at 64
     (the most the path tracer takes) the gradient must equal the FP64 one over the same paths."""
     import torch
 
@@ -281,8 +281,8 @@ def test_oracle_adjoint_furnace_closed_form(cq, oracle, bounces):
 @pytest.mark.gpu
 @pytest.mark.parametrize("bounces", [7, 8, 17, 64])
 def test_gpu_adjoint_furnace_deep_paths(cq, oracle, gpu_ctx, bounces):
-    """Every furnace path has bounces + 1 vertices, so the adjoint's backward sweep runs 1 to 9
-    segments of 8 vertices in every lane: the gradient equals the closed form and the FP64 oracle's
+    """Every furnace path has bounces + 1 vertices, so the adjoint's backward sweep runs 2 to 17
+    segments of 4 vertices in every lane: the gradient equals the closed form and the FP64 oracle's
     within 1e-4 relative."""
     import torch
 
