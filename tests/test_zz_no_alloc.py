@@ -30,7 +30,8 @@ def _last_error(cq):
 
 
 @pytest.mark.parametrize("cfg_kw", [dict(variant=2), dict(variant=2, bin_bits=3), dict(variant=3, bin_bits=4),
-                                    dict(variant=5, bin_bits=2), dict(variant=1), None])
+                                    dict(variant=5, bin_bits=2), dict(variant=1), dict(variant=2, bin_bits=3, stack_rows=4),
+                                    None])
 def test_noalloc_matches_intersect_twice(cq, gpu_ctx, vol128, cfg_kw):
     import torch
 
@@ -63,4 +64,22 @@ def test_noalloc_refuses_binning_past_its_scratch(cq, gpu_ctx, vol128):
     assert st == -7, st  # CQX_ERR_LIMIT
     assert "bins at most" in _last_error(cq)
     assert int(out.count_nonzero().item()) == 0  # nothing was launched
+    dv.free()
+
+
+def test_noalloc_at_its_binning_limit(cq, gpu_ctx, vol128):
+    """Exactly CQX_NOALLOC_MAX_BINNED_RAYS rays binned in the static scratch (its cell histograms,
+    order and cells all at their largest) give the records of cqx_intersect."""
+    import torch
+
+    dv = gpu_ctx.upload(vol128)
+    n = 1 << 22
+    r = torch.from_numpy(np.ascontiguousarray(cq.incoherent_rays(vol128, 7, 37, n))).cuda()
+    cfg = cq.LaunchCfg(variant=2, bin_bits=4)
+    want = cq.intersectVolume(gpu_ctx, dv, r, True, -1.0, cfg=cfg)
+    got = torch.full((n, 32), 0xAB, dtype=torch.uint8, device="cuda")
+    st = _noalloc(cq, gpu_ctx, dv, r, True, -1.0, cfg, got)
+    torch.cuda.synchronize()
+    assert st == 0, _last_error(cq)
+    assert torch.equal(got, want)
     dv.free()
