@@ -337,6 +337,28 @@ def test_gpu_autotuned_launch_is_exact(cq, oracle, gpu_ctx, vol128, rays128):
     ctx.close()
 
 
+def test_gpu_autotuned_launch_on_deep_volumes(cq, oracle):
+    """On SubDAGs 21-22 levels deep the autotuner also weighs short stacks (4, 8, 16 rows): its
+    choice, whatever it is, gives the oracle's records through the auto launch."""
+    import torch
+
+    from tests.raysets import deep_volumes
+
+    ctx = cq.Context(0)
+    for name, (hv, rays) in deep_volumes(cq).items():
+        dv = ctx.upload(hv)
+        many = np.ascontiguousarray(np.tile(rays, (max(1, (1 << 18) // len(rays)), 1)), dtype=np.float32)
+        r = torch.from_numpy(many).cuda()
+        best, ms = cq.autotune(ctx, dv, r, True, -1.0)
+        assert ms > 0 and best.stack_rows in (0, 4, 8, 16) and (best.stack_rows == 0 or best.variant in (2, 3))
+        g = cq.intersectVolume(ctx, dv, torch.from_numpy(rays).cuda(), True).cpu().numpy().view(cq.HIT_DTYPE)
+        o = oracle.intersect(hv.nodes, oracle.find_subdags(hv.nodes, hv.root), rays, True, -1.0, max_iters=1 << 22)
+        eq = records_equal(g.reshape(-1), o)
+        assert eq.all(), _report(f"{name} tuned {best.variant}/{best.stack_rows}", eq, rays, g.reshape(-1), o)
+        dv.free()
+    ctx.close()
+
+
 def test_gpu_deterministic(cq, gpu_ctx, dvol128, rays128):
     a = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
     b = _gpu(gpu_ctx, dvol128, rays128["incoherent"])
