@@ -197,6 +197,9 @@ __global__ void k_set_u32(uint32_t* __restrict__ x, uint32_t v)
     if (threadIdx.x == 0) *x = v;
 }
 
+// kRadiance: the render's shading pass (throughput and radiance kept per path). Without it, the
+// adjoint's replay: the same paths, recording each vertex's material (vmat, vstat) only.
+template <bool kRadiance>
 __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevParams* __restrict__ dp, uint64_t P, uint32_t depth,
                                                   const uint32_t* __restrict__ count_cur,
                                                   const cqx_ray* __restrict__ rays_cur,
@@ -218,7 +221,7 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
         p = depth == 0 ? uint32_t(i) : paths_cur[i];
         const cqx_hit h = hits[i];
         float b[3] = { 1.0f, 1.0f, 1.0f }, l[3] = { 0.0f, 0.0f, 0.0f };
-        if (depth > 0)
+        if (kRadiance && depth > 0)
             for (int a = 0; a < 3; a++) {
                 b[a] = beta[a * P + p];
                 l[a] = L[a * P + p];
@@ -246,12 +249,14 @@ __global__ void __launch_bounds__(cqx::kCompactBlock) k_pt_shade(const PtDevPara
                 rng.inc = ((uint64_t(p % d.spp)) << 1u) | 1u;
                 if (scatter(d, rng, pos, nrm, next.o, next.d)) {
                     rng_state[p] = rng.state;
-                    for (int a = 0; a < 3; a++) beta[a * P + p] = b[a] * d.albedo[m * 3 + a];
+                    if (kRadiance)
+                        for (int a = 0; a < 3; a++) beta[a * P + p] = b[a] * d.albedo[m * 3 + a];
                     cont = true;
                 }
             }
         }
-        for (int a = 0; a < 3; a++) L[a * P + p] = l[a];
+        if (kRadiance)
+            for (int a = 0; a < 3; a++) L[a * P + p] = l[a];
     }
     // compact the surviving rays
     const bool take[1] = { cont };
@@ -514,8 +519,10 @@ struct PtPlan {
     uint8_t* vstat = nullptr; // P path records (vertex count, escaped)
     float* partials = nullptr;
     size_t partial_floats = 0;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;        // the render: radiance, no path record
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_replay = nullptr; // the adjoint's replay: path record, no radiance
+    hipGraphExec_t exec_replay = nullptr;
 };
 
 void destroy_pt_plan(PtPlan* p)
@@ -523,6 +530,8 @@ void destroy_pt_plan(PtPlan* p)
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->exec_replay) (void)hipGraphExecDestroy(p->exec_replay);
+    if (p->graph_replay) (void)hipGraphDestroy(p->graph_replay);
     release(p->dparams);
     for (int k = 0; k < 2; k++) {
         release(p->rays[k]);
@@ -651,20 +660,24 @@ cqx_status get_plan(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_params* p
 
 // Allocates the wavefront buffers and captures generate -> (reset next count, intersect, shade) x
 // (bounces + 1) once into a graph.
-cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
+// The wavefront graph of the render (replay false) or of the adjoint's replay (replay true),
+// captured once per plan; the buffers they share are allocated with the first.
+cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p, bool replay)
 {
-    if (p->exec) return CQX_OK;
+    if (replay ? p->exec_replay : p->exec) return CQX_OK;
     const uint64_t P = p->P;
     cqx_status st = CQX_OK;
-    for (int k = 0; k < 2 && st == CQX_OK; k++) {
-        st = alloc(&p->rays[k], P);
-        if (st == CQX_OK) st = alloc(&p->paths[k], P);
+    if (!p->hits) {
+        for (int k = 0; k < 2 && st == CQX_OK; k++) {
+            st = alloc(&p->rays[k], P);
+            if (st == CQX_OK) st = alloc(&p->paths[k], P);
+        }
+        if (st == CQX_OK) st = alloc(&p->hits, P);
+        if (st == CQX_OK) st = alloc(&p->beta, 3 * P);
+        if (st == CQX_OK) st = alloc(&p->rng, P);
+        if (st == CQX_OK) st = alloc(&p->counts, 64);
     }
-    if (st == CQX_OK) st = alloc(&p->hits, P);
-    if (st == CQX_OK) st = alloc(&p->beta, 3 * P);
-    if (st == CQX_OK) st = alloc(&p->rng, P);
-    if (st == CQX_OK) st = alloc(&p->counts, 64);
-    if (st == CQX_OK) { // the adjoint's path record
+    if (st == CQX_OK && replay && !p->vstat) { // the adjoint's path record
         st = alloc(&p->vmat, (p->bounces + 1) * P);
         if (st == CQX_OK) st = alloc(&p->vstat, P);
     }
@@ -682,16 +695,21 @@ cqx_status ensure_wavefront(cqx_ctx* ctx, PtPlan* p)
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, cnt_next, 0u);
         cst = cqx::launch_intersect(ctx, p->vol, p->rays[cur], P, cnt_cur, CQX_SURFACE_PROPERTIES, p->max_footprint,
                                     p->hits, depth == 0 ? &p->cfg0 : &p->cfg, s);
-        hipLaunchKernelGGL(k_pt_shade, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur, p->rays[cur],
-                           p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt], cnt_next,
-                           p->vmat, p->vstat);
+        if (replay)
+            hipLaunchKernelGGL(k_pt_shade<false>, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur,
+                               p->rays[cur], p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt],
+                               cnt_next, p->vmat, p->vstat);
+        else
+            hipLaunchKernelGGL(k_pt_shade<true>, dim3(gc), dim3(cqx::kCompactBlock), 0, s, p->dparams, P, depth, cnt_cur,
+                               p->rays[cur], p->paths[cur], p->hits, p->beta, p->L, p->rng, p->rays[nxt], p->paths[nxt],
+                               cnt_next, nullptr, nullptr);
     }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(s, &g);
-    p->graph = g;
+    (replay ? p->graph_replay : p->graph) = g;
     if (cst != CQX_OK) return cst;
     if (e != hipSuccess || !g) return fail(CQX_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    HIP_TRY(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphInstantiate(replay ? &p->exec_replay : &p->exec, g, nullptr, nullptr, 0));
     return CQX_OK;
 }
 
@@ -737,7 +755,7 @@ cqx_status cqx_pathtrace(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_pt_para
     st = get_plan(ctx, vol, pp, cfg, &p);
     if (st != CQX_OK) return st;
     if (mode == 0) {
-        st = ensure_wavefront(ctx, p);
+        st = ensure_wavefront(ctx, p, false);
         if (st != CQX_OK) return st;
     }
     hipStream_t s = hipStream_t(stream_ptr);
@@ -783,7 +801,7 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     if (st != CQX_OK) return st;
     PtPlan* p = nullptr;
     st = get_plan(ctx, vol, pp, cfg, &p);
-    if (st == CQX_OK) st = ensure_wavefront(ctx, p);
+    if (st == CQX_OK) st = ensure_wavefront(ctx, p, true);
     if (st != CQX_OK) return st;
     const uint32_t block = kAdjointBlock;
     const uint32_t blocks =
@@ -800,10 +818,10 @@ cqx_status cqx_pathtrace_adjoint(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx
     }
     hipStream_t s = hipStream_t(stream_ptr);
     HIP_TRY(hipMemcpyAsync(p->dparams, &host, sizeof(host), hipMemcpyHostToDevice, s));
-    // replay the paths (the wavefront graph records each path's vertices), then sweep them backwards
+    // replay the paths (the replay graph records each path's vertices), then sweep them backwards
     {
         const cqx::TraceRange replay("adjoint: path replay (wavefront graph)");
-        HIP_TRY(hipGraphLaunch(p->exec, s));
+        HIP_TRY(hipGraphLaunch(p->exec_replay, s));
     }
     {
         const cqx::TraceRange sweep("adjoint: backward sweep and reduction");
