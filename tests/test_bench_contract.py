@@ -50,5 +50,11 @@ def test_traffic_profile_is_consistent():
         assert p["launch"] == bench.launch_string({k: int(v) for k, v in cfg.items()})
         traffic, l2, prof = bench.traffic_from_profiles(p["launch"])
         assert traffic == p["hbm_bytes_per_launch"] and l2 == p["l2_hit_rate"] and prof
+        # the launch the bench would time instead of an unprofiled tuned one parses into a LaunchCfg
+        import cubiquity_amd as cq
+
+        c = bench.launch_cfg(cq, p["launch"])
+        assert bench.launch_string({k: getattr(c, k) for k in bench.LAUNCH_KEYS + ("stack_rows",)}) == p["launch"]
+    assert bench.profiled_launches() == [p["launch"] for p in d["launches"]]
     assert bench.traffic_from_profiles("variant=3,block=128,waves_per_cu=7,regroup_interval=2,refill_lanes=3,"
                                        "lds_nodes=0,bin_bits=0") == (None, None, None)
