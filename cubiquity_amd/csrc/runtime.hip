@@ -337,16 +337,29 @@ __global__ void __launch_bounds__(256) k_intersect_pair(SingleSource src, uint64
 // of its contiguous range of rays in LDS and writes its histogram; one thread per cell turns the
 // histograms into each workgroup's offset within the cell and the cell's total; the totals are
 // scanned; each workgroup then places its rays from its own offsets (LDS atomics only).
+// The grid covers the rays' origins: the bounds of a sample of them, within the box of the
+// volume's SubDAGs. Edits past a volume's bounds grow that box (new SubDAGs in other octants, a
+// taller one in the first), and a grid over the whole box would put rays that start inside the old
+// bounds into a few of its cells.
 constexpr uint32_t kMaxBinBits = 4;
 constexpr uint32_t kMaxCells = 1u << (3 * kMaxBinBits); // 4096
 constexpr uint32_t kMaxBinBlocks = 256;
 constexpr uint32_t kBinBlock = 1024;   // threads per workgroup of the count and scatter kernels
 constexpr uint32_t kBinMinRange = 16384; // rays per workgroup at least
 
+constexpr uint32_t kBoundsBlocks = 256;      // workgroups of the origin-bounds pass, at most
+constexpr uint32_t kBoundsSample = 1u << 20; // about this many origins are read for the bounds
+
 struct BinGrid {
     float lo[3];
     float scale[3]; // cells per unit length along each axis
     uint32_t bits;  // 2^bits cells per axis
+};
+
+// The union of the boxes of the non-empty SubDAGs.
+struct BinBox {
+    float lo[3], hi[3];
+    uint32_t bits;
 };
 
 __device__ __forceinline__ uint32_t bin_cell(const BinGrid& g, const float* r)
@@ -364,12 +377,84 @@ __device__ __forceinline__ uint32_t bin_cell(const BinGrid& g, const float* r)
     return c; // < 2^(3 bits)
 }
 
+// Minimum and maximum of x, y, z over the workgroup's share of the origins of rays 0, stride,
+// 2 * stride, ...: partial[6 * blockIdx.x + (0..2 minima, 3..5 maxima)]. NaN coordinates are
+// skipped (fminf and fmaxf return the other operand); a share with none leaves (+inf, -inf).
+__global__ void __launch_bounds__(256) k_ray_bounds(const cqx_ray* __restrict__ rays, uint64_t n, uint64_t stride,
+                                                    float* __restrict__ partial)
+{
+    __shared__ float part[4][6];
+    float v[6] = { INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY };
+    const uint64_t step = uint64_t(gridDim.x) * blockDim.x * stride;
+    for (uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) * stride; i < n; i += step) {
+        const float* r = reinterpret_cast<const float*>(rays + i);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            v[a] = fminf(v[a], r[a]);
+            v[3 + a] = fmaxf(v[3 + a], r[a]);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], off));
+            v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], off));
+        }
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0)
+        for (int k = 0; k < 6; k++) part[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t k = threadIdx.x;
+        float x = part[0][k];
+        for (uint32_t j = 1; j < blockDim.x / 64; j++) x = k < 3 ? fminf(x, part[j][k]) : fmaxf(x, part[j][k]);
+        partial[6 * blockIdx.x + k] = x;
+    }
+}
+
+// The grid (one wave): per axis, the origins' bounds within the SubDAG box, or the whole box along
+// an axis where that is empty or a single point (no sampled origin in the box, or all at one
+// coordinate).
+__global__ void __launch_bounds__(64) k_bin_grid(const float* __restrict__ partial, uint32_t blocks, BinBox box,
+                                                 BinGrid* __restrict__ grid)
+{
+    float v[6] = { INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY };
+    for (uint32_t b = threadIdx.x; b < blocks; b += 64)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            v[a] = fminf(v[a], partial[6 * b + a]);
+            v[3 + a] = fmaxf(v[3 + a], partial[6 * b + 3 + a]);
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], off));
+            v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], off));
+        }
+    if (threadIdx.x != 0) return;
+    BinGrid g;
+    g.bits = box.bits;
+    for (int a = 0; a < 3; a++) {
+        float l = fmaxf(v[a], box.lo[a]), h = fminf(v[3 + a], box.hi[a]);
+        if (!(l < h)) {
+            l = box.lo[a];
+            h = box.hi[a];
+        }
+        g.lo[a] = l;
+        g.scale[a] = float(double(1u << box.bits) / (double(h) - double(l)));
+    }
+    *grid = g;
+}
+
 // Cell of every ray of the workgroup's range [blockIdx.x * range, ...), and the range's cell
 // histogram, hist[blockIdx.x * ncells + cell].
 __global__ void __launch_bounds__(kBinBlock) k_bin_count(const cqx_ray* __restrict__ rays, uint64_t n, uint64_t range,
-                                                         BinGrid g, uint32_t ncells, uint16_t* __restrict__ cells,
-                                                         uint32_t* __restrict__ hist)
+                                                         const BinGrid* __restrict__ grid, uint32_t ncells,
+                                                         uint16_t* __restrict__ cells, uint32_t* __restrict__ hist)
 {
+    const BinGrid g = *grid;
     __shared__ uint32_t h[kMaxCells];
     for (uint32_t c = threadIdx.x; c < ncells; c += blockDim.x) h[c] = 0u;
     __syncthreads();
@@ -444,9 +529,9 @@ __global__ void __launch_bounds__(kBinBlock) k_bin_scatter(const uint16_t* __res
     for (uint64_t i = first + threadIdx.x; i < last; i += blockDim.x) order[atomicAdd(&cur[cells[i]], 1u)] = uint32_t(i);
 }
 
-// The binning grid: 2^bits cells per axis over the union of the boxes of the non-empty SubDAGs.
-// False when every SubDAG is empty (nothing to bin for).
-bool bin_grid(const cqx_dvolume* vol, uint32_t bits, BinGrid& g)
+// The box the binning grid lies in: the union of the boxes of the non-empty SubDAGs. False when
+// every SubDAG is empty (nothing to bin for).
+bool bin_box(const cqx_dvolume* vol, uint32_t bits, BinBox& b)
 {
     double lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
     bool any = false;
@@ -461,16 +546,18 @@ bool bin_grid(const cqx_dvolume* vol, uint32_t bits, BinGrid& g)
         any = true;
     }
     if (!any) return false;
-    g.bits = bits;
+    b.bits = bits;
     for (int a = 0; a < 3; a++) {
-        g.lo[a] = float(lo[a]);
-        g.scale[a] = float(std::ldexp(1.0, int(bits)) / (hi[a] - lo[a]));
+        b.lo[a] = float(lo[a]);
+        b.hi[a] = float(hi[a]);
     }
     return true;
 }
 
 // Binning buffers: kMaxCells counts, kMaxBinBlocks x kMaxCells histogram entries, then n order
 // entries (u32) and n cells (u16).
+static_assert(sizeof(BinGrid) <= kMaxCells * 4, "the grid fits the counts area");
+static_assert(kBoundsBlocks * 6 <= kMaxBinBlocks * kMaxCells, "the bounds partials fit the histogram area");
 size_t bin_bytes(uint64_t n);
 
 cqx_status ensure_bins(cqx_ctx* ctx, uint64_t n)
@@ -494,7 +581,7 @@ size_t bin_bytes(uint64_t n)
     return size_t(kMaxCells) * 4 + size_t(kMaxBinBlocks) * kMaxCells * 4 + size_t(n) * 4 + size_t(n) * 2;
 }
 
-cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid& g, hipStream_t stream,
+cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinBox& box, hipStream_t stream,
                     const cqx::ScratchView* sv, const uint32_t*& order)
 {
     void* bins = nullptr;
@@ -511,11 +598,19 @@ cqx_status bin_rays(cqx_ctx* ctx, const cqx_ray* rays, uint64_t n, const BinGrid
     uint32_t* hist = counts + kMaxCells;
     uint32_t* ord = hist + size_t(kMaxBinBlocks) * kMaxCells;
     uint16_t* cells = reinterpret_cast<uint16_t*>(ord + n);
-    const uint32_t ncells = 1u << (3 * g.bits);
+    const uint32_t ncells = 1u << (3 * box.bits);
+    // the bounds pass's partials use the histogram area and the grid the counts area: both are
+    // read before the kernels that fill those areas run
+    float* partial = reinterpret_cast<float*>(hist);
+    BinGrid* grid = reinterpret_cast<BinGrid*>(counts);
+    const uint64_t stride = std::max<uint64_t>(1, n / kBoundsSample);
+    const uint32_t bblocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(kBoundsBlocks, (n / stride + 255) / 256)));
+    hipLaunchKernelGGL(k_ray_bounds, dim3(bblocks), dim3(256), 0, stream, rays, n, stride, partial);
+    hipLaunchKernelGGL(k_bin_grid, dim3(1), dim3(64), 0, stream, partial, bblocks, box, grid);
     const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>(kMaxBinBlocks, n / kBinMinRange));
     const uint64_t range = (n + groups - 1) / groups;
-    hipLaunchKernelGGL(k_bin_count, dim3(uint32_t(groups)), dim3(kBinBlock), 0, stream, rays, n, range, g, ncells, cells,
-                       hist);
+    hipLaunchKernelGGL(k_bin_count, dim3(uint32_t(groups)), dim3(kBinBlock), 0, stream, rays, n, range, grid, ncells,
+                       cells, hist);
     hipLaunchKernelGGL(k_bin_columns, dim3((ncells + 255) / 256), dim3(256), 0, stream, hist, uint32_t(groups), ncells,
                        counts);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, stream, counts, ncells);
@@ -649,9 +744,9 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
         if (variant < 2 || variant > 5) return fail(CQX_ERR_INVALID_ARG, "ray binning needs a persistent variant (2-5)");
         if (n_dev) return fail(CQX_ERR_INVALID_ARG, "ray binning needs the ray count on the host");
         if (n > 0xffffffffull) return fail(CQX_ERR_LIMIT, "ray binning takes fewer than 2^32 rays");
-        BinGrid g;
-        if (bin_grid(vol, bin_bits, g)) {
-            const cqx_status bst = bin_rays(ctx, rays, n, g, stream, sv, order);
+        BinBox box;
+        if (bin_box(vol, bin_bits, box)) {
+            const cqx_status bst = bin_rays(ctx, rays, n, box, stream, sv, order);
             if (bst != CQX_OK) return bst;
         }
     }

@@ -76,6 +76,25 @@ def test_gpu_binned_ragged_counts(cq, gpu_ctx, dvol128, vol128):
         assert records_equal(g, full[:n]).all(), n
 
 
+def test_gpu_binned_grid_follows_the_origins(cq, gpu_ctx, dvol128, vol128):
+    """The binning grid spans the origins' bounds within the SubDAG box: origins confined to one
+    corner of the volume, origins sharing one coordinate (a flat grid axis), and one shared origin
+    (every axis falls back to the box) give the records of unbinned launches."""
+    rays = cq.incoherent_rays(vol128, 7, 12, 40000)
+    corner = rays.copy()
+    corner[:, :3] = 1.0 + np.mod(np.abs(corner[:, :3]), 15.0)
+    flat = rays.copy()
+    flat[:, 1] = 64.0
+    point = rays.copy()
+    point[:, :3] = np.float32([40.5, 70.25, 90.0])
+    for name, r in (("corner", corner), ("flat", flat), ("point", point)):
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        full = _gpu(gpu_ctx, dvol128, r, True, -1.0, cq.LaunchCfg(variant=2))
+        for bits in (2, 4):
+            g = _gpu(gpu_ctx, dvol128, r, True, -1.0, cq.LaunchCfg(variant=3, bin_bits=bits))
+            assert records_equal(g, full).all(), (name, bits)
+
+
 def test_gpu_binning_rejects_bad_configs(cq, gpu_ctx, dvol128, rays128):
     import torch
 
