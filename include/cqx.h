@@ -322,7 +322,8 @@ typedef struct {
                                   LDS (the top of a breadth-first store; 0 = 512) */
     uint32_t bin_bits;         /* variants 2-5, cqx_intersect only: 0 = rays are taken in array order;
                                   1..4 = a prepass first bins the rays by origin into 2^(3 bin_bits)
-                                  Morton-ordered cells over the volume's SubDAG bounds, and the
+                                  Morton-ordered cells over the origins' bounds (sampled, within the
+                                  volume's SubDAG bounds), and the
                                   persistent waves take them cell by cell (each XCD a contiguous run
                                   of cells), so that rays in flight together read nearby nodes.
                                   Results are the same, in the caller's order. n < 2^32. */
