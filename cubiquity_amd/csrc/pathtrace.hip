@@ -305,6 +305,7 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
 {
     extern __shared__ uint32_t lds_stack[];
     const PtDevParams& d = *dp;
+    const uint32_t occ = occupied_octants(T.sub);
     const uint64_t stride_all = uint64_t(gridDim.x) * blockDim.x;
     for (uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < P; p += stride_all) {
         Pcg rng;
@@ -314,7 +315,7 @@ __global__ void __launch_bounds__(256) k_pt_megakernel(const PtDevParams* __rest
         float b[3] = { 1.0f, 1.0f, 1.0f }, l[3] = { 0.0f, 0.0f, 0.0f };
         for (uint32_t depth = 0;; depth++) {
             cqx_hit h;
-            trace_ray(T, o[0], o[1], o[2], dir[0], dir[1], dir[2], lds_stack + threadIdx.x, blockDim.x, h);
+            trace_ray(T, o[0], o[1], o[2], dir[0], dir[1], dir[2], lds_stack + threadIdx.x, blockDim.x, h, occ);
             if (h.info & CQX_HIT_ANOMALY) break;
             if (!(h.info & CQX_HIT_FLAG)) {
                 for (int a = 0; a < 3; a++) l[a] += b[a] * d.sky[a];

@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(256) k_intersect_lane(TraceParams P, const cqx
 #pragma unroll
     for (int k = 0; k < 6; k++) v[k] = __builtin_nontemporal_load(r + k);
     cqx_hit h;
-    trace_ray(P, v[0], v[1], v[2], v[3], v[4], v[5], lds_stack + threadIdx.x, blockDim.x, h);
+    trace_ray(P, v[0], v[1], v[2], v[3], v[4], v[5], lds_stack + threadIdx.x, blockDim.x, h, occupied_octants(P.sub));
     const uint32_t* src = reinterpret_cast<const uint32_t*>(&h);
     uint32_t* dst = reinterpret_cast<uint32_t*>(hits + i);
 #pragma unroll
@@ -182,6 +182,9 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
 
     RayState S;
     S.phase = kIdle;
+    // single-volume launches pass over the empty octants without reading their SubDAGs
+    uint32_t occ = 0xffu;
+    if constexpr (Source::kUniform) occ = occupied_octants(src.P.sub);
 
     for (;;) {
         // Lanes are busy while they walk a SubDAG. Between SubDAGs, finished or idle, they wait
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
         const uint64_t walking = __ballot(S.phase == kWalk);
         const uint32_t nwait = 64u - uint32_t(__popcll(walking));
         if (nwait >= refill || walking == 0) {
-            enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask);
+            enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask, occ);
             if (is_finished(S.phase)) {
                 cqx_hit out;
                 emit_record(uniform_params(src, P), S, out);
@@ -221,7 +224,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                     const float* r;
                     src.fetch(k, P, r, dst);
                     setup_from(S, r);
-                    enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask);
+                    enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask, occ);
                 }
             }
             const uint64_t nidle = uint64_t(__popcll(idle));

@@ -203,13 +203,23 @@ __device__ __forceinline__ uint32_t stack_row(int32_t h, uint32_t smask)
     return kShort ? uint32_t(h) & smask : uint32_t(h);
 }
 
+// Bit i set when SubDAG i is not empty.
+__device__ __forceinline__ uint32_t occupied_octants(const cqx_subdag* sub)
+{
+    uint32_t occ = 0;
+    for (uint32_t i = 0; i < 8; i++) occ |= (sub[i].node > 0 ? 1u : 0u) << i;
+    return occ;
+}
+
 // Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
-// (:220-250), or move on to the next octant.
+// (:220-250), or move on to the next octant. `occ`: bit i clear when SubDAG i is known to be empty
+// (its octant is passed over without reading it; all bits set reads every SubDAG, the same result).
 template <bool kCache = false, bool kShort = false>
 __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                             uint32_t smask = 0)
+                                             uint32_t smask = 0, uint32_t occ = 0xffu)
 {
-    const cqx_subdag sd = P.sub[S.oct ^ S.sign_bits];
+    const uint32_t o = S.oct ^ S.sign_bits;
+    const cqx_subdag sd = ((occ >> o) & 1u) ? P.sub[o] : cqx_subdag{};
     if (sd.node > 0) {
         const int32_t sbx = S.sign_bits & 1u, sby = (S.sign_bits >> 1) & 1u, sbz = (S.sign_bits >> 2) & 1u;
         const int32_t size = int32_t(1u << sd.height);
@@ -373,18 +383,19 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
 // the ray enters, or to the end of its walk. At most 8 octants are tried.
 template <bool kCache = false, bool kShort = false>
 __device__ __forceinline__ void enter_octants(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                              uint32_t smask = 0)
+                                              uint32_t smask = 0, uint32_t occ = 0xffu)
 {
     if (S.phase == kLeave) octant_advance(S);
-    while (S.phase == kOctant) octant_enter<kCache, kShort>(P, S, stack, stride, smask);
+    while (S.phase == kOctant) octant_enter<kCache, kShort>(P, S, stack, stride, smask, occ);
 }
 
 // One traversal step of a lane whose walk is in progress.
 template <bool kCache = false>
-__device__ __forceinline__ void step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride)
+__device__ __forceinline__ void step(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                     uint32_t occ = 0xffu)
 {
     if (S.phase == kWalk) walk_step<kCache>(P, S, stack, stride);
-    else enter_octants<kCache>(P, S, stack, stride);
+    else enter_octants<kCache>(P, S, stack, stride, 0, occ);
 }
 
 // Record of a finished ray (phase kHit, kMiss or kAnomaly). For a hit, the child-box entry
@@ -453,11 +464,12 @@ __device__ __forceinline__ void emit_record(const TraceParams& P, RayState& S, c
 
 // Runs one ray to completion (state machine driven in a loop).
 __device__ __forceinline__ void trace_ray(const TraceParams& P, float rox, float roy, float roz, float rdx, float rdy,
-                                          float rdz, uint32_t* stack, uint32_t stride, cqx_hit& out)
+                                          float rdz, uint32_t* stack, uint32_t stride, cqx_hit& out,
+                                          uint32_t occ = 0xffu)
 {
     RayState S;
     ray_setup(S, rox, roy, roz, rdx, rdy, rdz);
-    while (!is_finished(S.phase)) step(P, S, stack, stride);
+    while (!is_finished(S.phase)) step(P, S, stack, stride, occ);
     emit_record(P, S, out);
 }
 

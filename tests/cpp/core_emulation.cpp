@@ -149,6 +149,7 @@ int main(int argc, char** argv)
         uint64_t idx[64] = {};
         for (RayState& s : S) s.phase = kIdle;
         uint64_t next = 0;
+        const uint32_t occ = occupied_octants(P.sub); // as the persistent kernel passes empty octants over
         // census: lane slots, service passes, steps per path (per lane; per wave step with the path)
         enum { kD, kA, kP, kL, kH, kPaths };
         uint64_t slots_walk = 0, slots_wait = 0, services = 0, wave_steps = 0, lane_paths[kPaths] = {},
@@ -159,8 +160,8 @@ int main(int argc, char** argv)
             if (64 - walking >= refill || walking == 0) {
                 services++;
                 for (int l = 0; l < 64; l++) {
-                    if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask);
-                    else enter_octants(P, S[l], &stack[l], 64);
+                    if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask, occ);
+                    else enter_octants(P, S[l], &stack[l], 64, 0, occ);
                 }
                 for (int l = 0; l < 64; l++)
                     if (is_finished(S[l].phase)) {
@@ -172,8 +173,8 @@ int main(int argc, char** argv)
                         const float* r = rays + 6 * next;
                         idx[l] = next++;
                         ray_setup(S[l], r[0], r[1], r[2], r[3], r[4], r[5]);
-                        if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask);
-                        else enter_octants(P, S[l], &stack[l], 64);
+                        if (short_stack) enter_octants<false, true>(P, S[l], &stack[l], 64, smask, occ);
+                        else enter_octants(P, S[l], &stack[l], 64, 0, occ);
                     }
                 int live = 0;
                 for (const RayState& s : S) live += s.phase != kIdle;
