@@ -144,6 +144,9 @@ struct BatchSource {
 __device__ __forceinline__ const TraceParams& uniform_params(const SingleSource& s, const TraceParams&) { return s.P; }
 __device__ __forceinline__ const TraceParams& uniform_params(const BatchSource&, const TraceParams& lane) { return lane; }
 
+// Static LDS of k_intersect_persistent besides its dynamic stacks: the copy of the 8 SubDAGs.
+constexpr size_t kSubdagCopyBytes = 8 * sizeof(cqx_subdag);
+
 template <typename Source, bool kCache, bool kTop = false, bool kShort = false>
 __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64_t n, const uint32_t* __restrict__ n_dev,
                                                               unsigned long long* __restrict__ counters,
@@ -182,9 +185,19 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
 
     RayState S;
     S.phase = kIdle;
-    // single-volume launches pass over the empty octants without reading their SubDAGs
+    // single-volume launches pass over the empty octants without reading their SubDAGs, and read
+    // the others from an LDS copy
     uint32_t occ = 0xffu;
-    if constexpr (Source::kUniform) occ = occupied_octants(src.P.sub);
+    __shared__ cqx_subdag lds_sub[8];
+    if constexpr (Source::kUniform) {
+        occ = occupied_octants(src.P.sub);
+        if (threadIdx.x < 8) lds_sub[threadIdx.x] = src.P.sub[threadIdx.x];
+        __syncthreads();
+    }
+    const auto enter = [&](RayState& R) {
+        if constexpr (Source::kUniform) enter_octants_from<kCache, kShort>(lds_sub, src.P, R, stack, stride, smask, occ);
+        else enter_octants<kCache, kShort>(uniform_params(src, P), R, stack, stride, smask);
+    };
 
     for (;;) {
         // Lanes are busy while they walk a SubDAG. Between SubDAGs, finished or idle, they wait
@@ -192,7 +205,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
         const uint64_t walking = __ballot(S.phase == kWalk);
         const uint32_t nwait = 64u - uint32_t(__popcll(walking));
         if (nwait >= refill || walking == 0) {
-            enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask, occ);
+            enter(S);
             if (is_finished(S.phase)) {
                 cqx_hit out;
                 emit_record(uniform_params(src, P), S, out);
@@ -224,7 +237,7 @@ __global__ void __launch_bounds__(256) k_intersect_persistent(Source src, uint64
                     const float* r;
                     src.fetch(k, P, r, dst);
                     setup_from(S, r);
-                    enter_octants<kCache, kShort>(uniform_params(src, P), S, stack, stride, smask, occ);
+                    enter(S);
                 }
             }
             const uint64_t nidle = uint64_t(__popcll(idle));
@@ -781,6 +794,8 @@ cqx_status launch_intersect(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray*
             if (lds_total > size_t(ctx->lds_per_block))
                 return fail(CQX_ERR_LIMIT, "stack and %u LDS nodes need %zu B of LDS per block", top_nodes, lds_total);
         }
+        if (lds_total + kSubdagCopyBytes > size_t(ctx->lds_per_block))
+            return fail(CQX_ERR_LIMIT, "stack and SubDAG copy need %zu B of LDS per block", lds_total + kSubdagCopyBytes);
         SingleSource src{ P, rays, hits, order };
         hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, stream, counters);
         if (variant == 2 && short_stack)
@@ -1097,7 +1112,8 @@ cqx_status cqx_intersect_batched(cqx_ctx* ctx, const cqx_batch* batches, uint32_
         }
     }
     const size_t lds = size_t(rows) * block * sizeof(uint32_t);
-    if (lds > size_t(ctx->lds_per_block)) return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds);
+    if (lds + kSubdagCopyBytes > size_t(ctx->lds_per_block))
+        return fail(CQX_ERR_LIMIT, "stack needs %zu B of LDS per block", lds + kSubdagCopyBytes);
     hipStream_t s = hipStream_t(stream_ptr);
     // The descriptor table is per context: wait for an earlier batched launch on this stream.
     HIP_TRY(hipStreamSynchronize(s));

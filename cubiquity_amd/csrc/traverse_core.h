@@ -214,12 +214,14 @@ __device__ __forceinline__ uint32_t occupied_octants(const cqx_subdag* sub)
 // Enter the current octant's SubDAG (raytracing.cpp:443-458) and initialise the ESVO walk
 // (:220-250), or move on to the next octant. `occ`: bit i clear when SubDAG i is known to be empty
 // (its octant is passed over without reading it; all bits set reads every SubDAG, the same result).
+// `tab`: the SubDAGs read, P.sub or a copy of it (a kernel's LDS copy).
 template <bool kCache = false, bool kShort = false>
-__device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
-                                             uint32_t smask = 0, uint32_t occ = 0xffu)
+__device__ __forceinline__ void octant_enter_from(const cqx_subdag* tab, const TraceParams& P, RayState& S,
+                                                  uint32_t* stack, uint32_t stride, uint32_t smask = 0,
+                                                  uint32_t occ = 0xffu)
 {
     const uint32_t o = S.oct ^ S.sign_bits;
-    const cqx_subdag sd = ((occ >> o) & 1u) ? P.sub[o] : cqx_subdag{};
+    const cqx_subdag sd = ((occ >> o) & 1u) ? tab[o] : cqx_subdag{};
     if (sd.node > 0) {
         const int32_t sbx = S.sign_bits & 1u, sby = (S.sign_bits >> 1) & 1u, sbz = (S.sign_bits >> 2) & 1u;
         const int32_t size = int32_t(1u << sd.height);
@@ -257,6 +259,13 @@ __device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, 
         }
     }
     octant_advance(S);
+}
+
+template <bool kCache = false, bool kShort = false>
+__device__ __forceinline__ void octant_enter(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
+                                             uint32_t smask = 0, uint32_t occ = 0xffu)
+{
+    octant_enter_from<kCache, kShort>(P.sub, P, S, stack, stride, smask, occ);
 }
 
 // The child read of a walk step (raytracing.cpp:265): slot cid ^ signBits of the current node.
@@ -382,11 +391,19 @@ __device__ __forceinline__ void walk_step(const TraceParams& P, RayState& S, uin
 // Moves a lane that is between SubDAGs (phase kLeave or kOctant) into the walk of the next SubDAG
 // the ray enters, or to the end of its walk. At most 8 octants are tried.
 template <bool kCache = false, bool kShort = false>
+__device__ __forceinline__ void enter_octants_from(const cqx_subdag* tab, const TraceParams& P, RayState& S,
+                                                   uint32_t* stack, uint32_t stride, uint32_t smask = 0,
+                                                   uint32_t occ = 0xffu)
+{
+    if (S.phase == kLeave) octant_advance(S);
+    while (S.phase == kOctant) octant_enter_from<kCache, kShort>(tab, P, S, stack, stride, smask, occ);
+}
+
+template <bool kCache = false, bool kShort = false>
 __device__ __forceinline__ void enter_octants(const TraceParams& P, RayState& S, uint32_t* stack, uint32_t stride,
                                               uint32_t smask = 0, uint32_t occ = 0xffu)
 {
-    if (S.phase == kLeave) octant_advance(S);
-    while (S.phase == kOctant) octant_enter<kCache, kShort>(P, S, stack, stride, smask, occ);
+    enter_octants_from<kCache, kShort>(P.sub, P, S, stack, stride, smask, occ);
 }
 
 // One traversal step of a lane whose walk is in progress.
