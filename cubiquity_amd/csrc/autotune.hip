@@ -5,7 +5,9 @@
 // waves per CU, the regroup interval (traversal steps between refill checks) and the refill
 // threshold (idle lanes that trigger a refill). The binning prepass is inside the timed span. Each
 // candidate is timed with HIP events (median of 3 after a warm-up); the search is coordinate
-// descent from the built-in defaults, two sweeps. Results are cached in the context per
+// descent, two sweeps, from variant 2, 256 threads, 32 waves/CU, regroup 16, refill 16 (the
+// built-in defaults with regroup 16: started at regroup 8, the first variant sweep could settle in
+// variant 3, ~3 % slower on the 1024^3 incoherent workload, and not leave it). Results are cached in the context per
 // (SubDAG height, batch-size class = floor(log2 n)) and used by every cqx_intersect whose
 // configuration is left at "auto" (cfg == NULL or cfg->variant == 0), also for volumes and batches
 // within 2 of a tuned height and class (tuned_cfg).
@@ -104,7 +106,7 @@ cqx_status cqx_autotune(cqx_ctx* ctx, const cqx_dvolume* vol, const cqx_ray* ray
     cur.variant = 2;
     cur.block = 256;
     cur.waves_per_cu = 32;
-    cur.regroup_interval = 8;
+    cur.regroup_interval = 16;
     cur.refill_lanes = 16;
     float cur_ms = 0;
     cqx_status st = time_cfg(ctx, vol, rays, n, flags, max_footprint, hits, cur, s, cur_ms);
