@@ -181,13 +181,22 @@ def cpu_reference_shading(cq, W):
 
 
 def cpu_model():
+    """/proc/cpuinfo's model name with its family, model and stepping numbers (some hosts give only
+    a generic name such as "AMD EPYC")."""
+    fields = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            key, _, value = line.partition(":")
+            key = key.strip()
+            if key in ("model name", "cpu family", "model", "stepping") and key not in fields:
+                fields[key] = value.strip()
+            if not line.strip() and fields:
+                break  # the first processor's block
     except OSError:
         pass
-    return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def timed(fn, reps, warmup=1):
