@@ -9,9 +9,10 @@
 //                           global counter with one atomic, handed out by ballot prefix
 //                           (variant 2); variant 3 also keeps the current node's 8 children in
 //                           registers
-//   k_bin_count / k_bin_scan / k_bin_scatter
+//   k_ray_bounds / k_bin_grid / k_bin_count / k_bin_columns / k_bin_scan / k_bin_scatter
 //                           optional prepass of the persistent variants: rays binned by origin
-//                           into a coarse Morton grid, handed out cell by cell
+//                           into a coarse Morton grid over the origins' bounds, handed out cell
+//                           by cell
 // All keep the traversal stack in LDS (one column per lane, one row per node height).
 #include "device_internal.h"
 
